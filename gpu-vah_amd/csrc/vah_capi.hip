@@ -1,0 +1,620 @@
+// vah_capi.hip -- host side of the C ABI declared in include/vah.h.
+//
+// One vah_ctx owns the device state of one lattice and a HIP stream.  A step
+// is two launches of the fused stage kernel (vah_stage.h); buffer roles
+// rotate on the host instead of copying:
+//   conserved  q^n, q*, q^{n+1}      three 15-field slabs, q^n <-> q^{n+1} per step
+//   velocity   u^n, u^{n-1}, u*      three 4-field slabs, (u^n, u^{n-1}) <- (old u^{n-1} slab, u^n)
+//   e, p       two buffers each, stage 1 writes the other one, stage 2 writes back
+// which is the data flow of rungeKutta2
+// (old_cpu-vah_code/src/FullyDiscreteKurganovTadmorScheme.cpp:738-792) with the
+// in-place e/p updates of the reference split into read and write buffers so
+// that a cell's neighbours are never overwritten while being read.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/vah.h"
+#include "vah_stage.h"
+
+using namespace vah;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+int fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                      \
+    do {                                                                                   \
+        hipError_t _e = (expr);                                                            \
+        if (_e != hipSuccess) return fail(VAH_ERR_HIP, "%s: %s", #expr, hipGetErrorString(_e)); \
+    } while (0)
+
+// ---- glibc-compatible powf(x, 0.25f) correction bitmap -------------------
+// Bit `off` is set when the host libm's powf differs from the correctly
+// rounded fourth root for x with bit pattern (124 << 23) + off.  The pattern
+// repeats every 4 binades (see conformal_T in vah_device.h).
+std::once_flag g_fix_once;
+std::vector<uint32_t> g_fix;
+
+void build_powf_fix() {
+    const uint32_t nbits = 1u << 25;
+    g_fix.assign(nbits / 32, 0u);
+    unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    const uint32_t words = nbits / 32, per = (words + nth - 1) / nth;
+    for (unsigned t = 0; t < nth; ++t) {
+        th.emplace_back([t, per, words] {
+            uint32_t w0 = t * per, w1 = std::min(words, w0 + per);
+            for (uint32_t w = w0; w < w1; ++w) {
+                uint32_t word = 0;
+                for (uint32_t bit = 0; bit < 32; ++bit) {
+                    uint32_t b = (124u << 23) + w * 32 + bit;
+                    float x;
+                    memcpy(&x, &b, 4);
+                    float g = powf(x, 0.25f);
+                    float r = (float)sqrt(sqrt((double)x));
+                    if (g != r) word |= 1u << bit;
+                }
+                g_fix[w] = word;
+            }
+        });
+    }
+    for (auto &x : th) x.join();
+}
+
+}  // namespace
+
+struct vah_ctx {
+    vah_lattice lat;
+    vah_hydro hydro;
+    vah_options opt;
+    int ncx, ncy, ncz;
+    long ncell, fs;
+    int device;
+    hipStream_t stream;
+    bool own_stream;
+    double *slab;
+    double *qset[3];  // conserved slabs
+    double *useq[3];  // velocity slabs
+    double *ebuf[2], *pbuf[2];
+    int qi, qs, qo;        // q^n, q*, q^{n+1}
+    int uc, up, us;        // u^n, u^{n-1}, u*
+    int ecur;              // e/p buffer holding the current level
+    unsigned int *counters;  // device, [0] NaN, [1] fallbacks
+    unsigned int counters_seen[2];
+    uint32_t *powf_fix;
+    double *eos_dev;       // p then T
+    Eos eos;
+    long long nstep;
+    bool have_state;
+    // launch geometry
+    int tiles_x, tiles_y, zchunks, kz, nblocks, grid;
+    // profiling
+    bool prof;
+    std::vector<hipEvent_t> ev;  // pairs per launch
+    std::vector<int> ev_kind;
+    double *red_buf;       // per-block partial reductions
+    int red_blocks;
+};
+
+namespace {
+
+bool lattice_ok(const vah_lattice *l) {
+    return l && l->nx > 0 && l->ny > 0 && l->nz > 0 && l->dx > 0 && l->dy > 0 && l->dz > 0 && l->dt > 0;
+}
+
+StageParams make_params(const vah_ctx *c, int stage, double t) {
+    StageParams P;
+    const double *const_q[3] = {c->qset[0], c->qset[1], c->qset[2]};
+    P.fs = c->fs;
+    P.nx = c->lat.nx;
+    P.ny = c->lat.ny;
+    P.nz = c->lat.nz;
+    P.ncx = c->ncx;
+    P.ncy = c->ncy;
+    P.sy = c->ncx;
+    P.sz = (long)c->ncx * c->ncy;
+    P.tiles_x = c->tiles_x;
+    P.tiles_y = c->tiles_y;
+    P.zchunks = c->zchunks;
+    P.kz = c->kz;
+    P.nblocks = c->nblocks;
+    P.dt = c->lat.dt;
+    P.dt_inv = 1.0 / c->lat.dt;
+    P.etabar = c->hydro.etabar;
+    P.inv_dx = 1.0 / c->lat.dx;
+    P.inv_dy = 1.0 / c->lat.dy;
+    P.inv_dz = 1.0 / c->lat.dz;
+    P.hx = 1 / c->lat.dx / 2;
+    P.hy = 1 / c->lat.dy / 2;
+    P.hz = 1 / c->lat.dz / 2;
+    P.eos = c->eos;
+    P.counters = c->counters;
+    const int e_in = stage == 1 ? c->ecur : 1 - c->ecur;
+    P.e_in = c->ebuf[e_in];
+    P.p_in = c->pbuf[e_in];
+    P.e_out = c->ebuf[1 - e_in];
+    P.p_out = c->pbuf[1 - e_in];
+    if (stage == 1) {
+        P.qin = const_q[c->qi];
+        P.qout = c->qset[c->qs];
+        P.qn = const_q[c->qi];
+        P.ucur = c->useq[c->uc];
+        P.uprev = c->useq[c->up];
+        P.uguess = c->useq[c->uc];
+        P.uout = c->useq[c->us];
+        P.t_src = t;
+        P.t_rec = t + c->lat.dt;
+    } else {
+        P.qin = const_q[c->qs];
+        P.qout = c->qset[c->qo];
+        P.qn = const_q[c->qi];
+        P.ucur = c->useq[c->us];
+        P.uprev = c->useq[c->uc];
+        P.uguess = c->useq[c->uc];
+        P.uout = c->useq[c->up];
+        P.t_src = t + c->lat.dt;
+        P.t_rec = t + c->lat.dt;
+    }
+    return P;
+}
+
+template <bool D3, bool S2, bool SH>
+void launch(const vah_ctx *c, const StageParams &P) {
+    hipLaunchKernelGGL((stage_kernel<D3, S2, SH>), dim3(c->grid), dim3(NTHREADS), 0, c->stream, P);
+}
+
+void launch_stage(const vah_ctx *c, int stage, const StageParams &P) {
+    const bool d3 = c->lat.nz > 1, sh = c->opt.flux_mode == VAH_FLUX_SHARED;
+    if (d3) {
+        if (stage == 1) sh ? launch<true, false, true>(c, P) : launch<true, false, false>(c, P);
+        else sh ? launch<true, true, true>(c, P) : launch<true, true, false>(c, P);
+    } else {
+        if (stage == 1) sh ? launch<false, false, true>(c, P) : launch<false, false, false>(c, P);
+        else sh ? launch<false, true, true>(c, P) : launch<false, true, false>(c, P);
+    }
+}
+
+int fill_ghosts(vah_ctx *c) {
+    long n = (long)c->lat.nx * c->lat.ny * c->lat.nz;
+    int blocks = (int)((n + 255) / 256);
+    hipLaunchKernelGGL(ghost_fill_kernel, dim3(blocks), dim3(256), 0, c->stream, c->qset[c->qi], c->ebuf[c->ecur],
+                       c->pbuf[c->ecur], c->useq[c->uc], c->fs, c->lat.nx, c->lat.ny, c->lat.nz, (long)c->ncx,
+                       (long)c->ncx * c->ncy, c->lat.nz > 1 ? 1 : 0);
+    HIP_TRY(hipGetLastError());
+    return VAH_OK;
+}
+
+inline long comp_index(const vah_ctx *c, long m) {
+    int i = (int)(m % c->lat.nx);
+    int j = (int)((m / c->lat.nx) % c->lat.ny);
+    int k = (int)(m / ((long)c->lat.nx * c->lat.ny));
+    return (long)(k + 2) * c->ncx * c->ncy + (long)(j + 2) * c->ncx + (i + 2);
+}
+
+// host EOS pressure for initial states (same lookup as the device)
+double host_eos_p(const vah_ctx *c, const std::vector<double> &tab, double e) {
+    if (c->opt.eos == VAH_EOS_CONFORMAL) return e / 3;
+    const vah_eos_table &T = c->opt.table;
+    double f = (log(e) - T.lne0) * (1.0 / T.h);
+    int last = T.n - 1;
+    if (!(f >= 0.0)) return tab[0] * (e / exp(T.lne0));
+    int i = (int)f;
+    if (i >= last) return tab[last] * (e / exp(T.lne0 + last / (1.0 / T.h)));
+    double w = f - (double)i;
+    return tab[i] + w * (tab[i + 1] - tab[i]);
+}
+
+// copy interior host arrays into the padded device layout of one slab
+int upload_fields(vah_ctx *c, double *dst_slab, int nfields, const double *src, std::vector<double> &stage_buf) {
+    long n = (long)c->lat.nx * c->lat.ny * c->lat.nz;
+    stage_buf.assign((size_t)nfields * c->fs, 0.0);
+    for (int f = 0; f < nfields; ++f)
+        for (long m = 0; m < n; ++m) stage_buf[(size_t)f * c->fs + comp_index(c, m)] = src[(size_t)f * n + m];
+    HIP_TRY(hipMemcpyAsync(dst_slab, stage_buf.data(), sizeof(double) * nfields * c->fs, hipMemcpyHostToDevice,
+                           c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return VAH_OK;
+}
+
+int download_fields(vah_ctx *c, const double *src_slab, int nfields, double *dst) {
+    long n = (long)c->lat.nx * c->lat.ny * c->lat.nz;
+    std::vector<double> buf((size_t)nfields * c->fs);
+    HIP_TRY(hipMemcpyAsync(buf.data(), src_slab, sizeof(double) * nfields * c->fs, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (int f = 0; f < nfields; ++f)
+        for (long m = 0; m < n; ++m) dst[(size_t)f * n + m] = buf[(size_t)f * c->fs + comp_index(c, m)];
+    return VAH_OK;
+}
+
+__global__ void reduce_kernel(const double *q, const double *e, const double *u, long fs, int nx, int ny, int nz,
+                              long sy, long sz, Eos eos, double *out) {
+    __shared__ double sm[4][256];
+    long n = (long)nx * ny * nz;
+    double mx_e = 0.0, mx_T = 0.0, sum = 0.0, mx_v = 0.0;
+    for (long m = (long)blockIdx.x * blockDim.x + threadIdx.x; m < n; m += (long)gridDim.x * blockDim.x) {
+        int ii = (int)(m % nx);
+        int jj = (int)((m / nx) % ny);
+        int kk = (int)(m / ((long)nx * ny));
+        long s = (long)(kk + 2) * sz + (long)(jj + 2) * sy + (ii + 2);
+        double ev = e[s];
+        mx_e = fmax(mx_e, ev);
+        mx_T = fmax(mx_T, eos_T(eos, ev));
+        sum += q[s];
+        double ut = u[s], ux = u[fs + s], uy = u[2 * fs + s];
+        mx_v = fmax(mx_v, __builtin_sqrt(ux * ux + uy * uy) / ut);
+    }
+    int t = threadIdx.x;
+    sm[0][t] = mx_e;
+    sm[1][t] = mx_T;
+    sm[2][t] = sum;
+    sm[3][t] = mx_v;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) {
+            sm[0][t] = fmax(sm[0][t], sm[0][t + w]);
+            sm[1][t] = fmax(sm[1][t], sm[1][t + w]);
+            sm[2][t] += sm[2][t + w];
+            sm[3][t] = fmax(sm[3][t], sm[3][t + w]);
+        }
+        __syncthreads();
+    }
+    if (t == 0)
+        for (int k = 0; k < 4; ++k) out[blockIdx.x * 4 + k] = sm[k][0];
+}
+
+}  // namespace
+
+extern "C" {
+
+void vah_default_options(vah_options *opt) {
+    if (!opt) return;
+    memset(opt, 0, sizeof *opt);
+    opt->eos = VAH_EOS_CONFORMAL;
+    opt->precision = VAH_FP64;
+    opt->flux_mode = VAH_FLUX_SHARED;
+    opt->device = 0;
+    opt->stream = nullptr;
+}
+
+const char *vah_last_error(void) { return g_err.c_str(); }
+
+const char *vah_build_info(void) { return "vah-mi355x gfx950 fp64 fused-KT-stage"; }
+
+int vah_create(const vah_lattice *lat, const vah_hydro *hydro, const vah_options *opt_in, vah_ctx **out) {
+    if (!out) return fail(VAH_ERR_ARG, "vah_create: out is NULL");
+    *out = nullptr;
+    if (!lattice_ok(lat)) return fail(VAH_ERR_ARG, "vah_create: lattice sizes and spacings must be positive");
+    if (!hydro || !(hydro->t0 > 0) || !(hydro->etabar > 0))
+        return fail(VAH_ERR_ARG, "vah_create: need t0 > 0 and eta/s > 0");
+    vah_options opt;
+    if (opt_in)
+        opt = *opt_in;
+    else
+        vah_default_options(&opt);
+    if (opt.precision != VAH_FP64) return fail(VAH_ERR_ARG, "vah_create: precision %d not built", opt.precision);
+    if (opt.flux_mode != VAH_FLUX_SHARED && opt.flux_mode != VAH_FLUX_PER_CELL)
+        return fail(VAH_ERR_ARG, "vah_create: unknown flux mode %d", opt.flux_mode);
+    if (opt.eos == VAH_EOS_TABLE && (opt.table.n < 2 || !opt.table.p || !opt.table.T || !(opt.table.h > 0)))
+        return fail(VAH_ERR_ARG, "vah_create: EOS table needs n >= 2 nodes, h > 0 and p, T arrays");
+    if (opt.eos != VAH_EOS_CONFORMAL && opt.eos != VAH_EOS_TABLE)
+        return fail(VAH_ERR_ARG, "vah_create: unknown EOS kind %d", opt.eos);
+
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= opt.device || opt.device < 0)
+        return fail(VAH_ERR_DEVICE, "vah_create: HIP device %d not available (%d found)", opt.device, ndev);
+    HIP_TRY(hipSetDevice(opt.device));
+
+    vah_ctx *c = new vah_ctx();
+    c->lat = *lat;
+    c->hydro = *hydro;
+    c->opt = opt;
+    c->device = opt.device;
+    c->ncx = lat->nx + 4;
+    c->ncy = lat->ny + 4;
+    c->ncz = lat->nz + 4;
+    c->ncell = (long)c->ncx * c->ncy * c->ncz;
+    c->fs = (c->ncell + 63) / 64 * 64;
+    c->qi = 0;
+    c->qs = 1;
+    c->qo = 2;
+    c->uc = 0;
+    c->up = 1;
+    c->us = 2;
+    c->ecur = 0;
+
+    auto cleanup = [&](int code) {
+        vah_destroy(c);
+        return code;
+    };
+    if (opt.stream) {
+        c->stream = (hipStream_t)opt.stream;
+        c->own_stream = false;
+    } else {
+        if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+            return cleanup(fail(VAH_ERR_HIP, "vah_create: stream creation failed"));
+        c->own_stream = true;
+    }
+    const size_t nslab = 3 * NQ + 4 + 12;
+    size_t bytes = sizeof(double) * c->fs * nslab;
+    if (hipMalloc(&c->slab, bytes) != hipSuccess)
+        return cleanup(fail(VAH_ERR_NOMEM, "vah_create: cannot allocate %.2f GB of device state", bytes / 1e9));
+    if (hipMemsetAsync(c->slab, 0, bytes, c->stream) != hipSuccess)
+        return cleanup(fail(VAH_ERR_HIP, "vah_create: memset failed"));
+    double *b = c->slab;
+    for (int k = 0; k < 3; ++k, b += NQ * c->fs) c->qset[k] = b;
+    for (int k = 0; k < 2; ++k, b += c->fs) c->ebuf[k] = b;
+    for (int k = 0; k < 2; ++k, b += c->fs) c->pbuf[k] = b;
+    for (int k = 0; k < 3; ++k, b += 4 * c->fs) c->useq[k] = b;
+
+    if (hipMalloc(&c->counters, 4 * sizeof(unsigned)) != hipSuccess ||
+        hipMemsetAsync(c->counters, 0, 4 * sizeof(unsigned), c->stream) != hipSuccess)
+        return cleanup(fail(VAH_ERR_NOMEM, "vah_create: counters"));
+
+    memset(&c->eos, 0, sizeof c->eos);
+    c->eos.kind = opt.eos == VAH_EOS_TABLE ? 1 : 0;
+    if (opt.eos == VAH_EOS_CONFORMAL) {
+        std::call_once(g_fix_once, build_powf_fix);
+        if (hipMalloc(&c->powf_fix, g_fix.size() * sizeof(uint32_t)) != hipSuccess ||
+            hipMemcpyAsync(c->powf_fix, g_fix.data(), g_fix.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+                           c->stream) != hipSuccess)
+            return cleanup(fail(VAH_ERR_NOMEM, "vah_create: powf table"));
+        c->eos.powf_fix = c->powf_fix;
+    } else {
+        const vah_eos_table &T = opt.table;
+        if (hipMalloc(&c->eos_dev, sizeof(double) * 2 * T.n) != hipSuccess)
+            return cleanup(fail(VAH_ERR_NOMEM, "vah_create: EOS table"));
+        std::vector<double> tab(2 * (size_t)T.n);
+        memcpy(tab.data(), T.p, sizeof(double) * T.n);
+        memcpy(tab.data() + T.n, T.T, sizeof(double) * T.n);
+        if (hipMemcpy(c->eos_dev, tab.data(), sizeof(double) * 2 * T.n, hipMemcpyHostToDevice) != hipSuccess)
+            return cleanup(fail(VAH_ERR_HIP, "vah_create: EOS upload"));
+        c->eos.n = T.n;
+        c->eos.lne0 = T.lne0;
+        c->eos.inv_h = 1.0 / T.h;
+        c->eos.e_first = exp(T.lne0);
+        c->eos.e_last = exp(T.lne0 + (T.n - 1) / (1.0 / T.h));
+        c->eos.p = c->eos_dev;
+        c->eos.T = c->eos_dev + T.n;
+        // keep a host copy of the pressure nodes for initial states
+        c->opt.table.p = nullptr;
+        c->opt.table.T = nullptr;
+    }
+
+    // launch geometry
+    c->tiles_x = (lat->nx + TX - 1) / TX;
+    c->tiles_y = (lat->ny + TY - 1) / TY;
+    c->kz = lat->nz > 1 ? std::min(8, lat->nz) : 1;
+    c->zchunks = (lat->nz + c->kz - 1) / c->kz;
+    c->nblocks = c->tiles_x * c->tiles_y * c->zchunks;
+    c->grid = (c->nblocks + 7) / 8 * 8;
+    c->red_blocks = 1024;
+    if (hipMalloc(&c->red_buf, sizeof(double) * 4 * c->red_blocks) != hipSuccess)
+        return cleanup(fail(VAH_ERR_NOMEM, "vah_create: reduction buffer"));
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return cleanup(fail(VAH_ERR_HIP, "vah_create: sync"));
+    *out = c;
+    return VAH_OK;
+}
+
+void vah_destroy(vah_ctx *c) {
+    if (!c) return;
+    // teardown: errors are not actionable here
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto &e : c->ev) (void)hipEventDestroy(e);
+    (void)hipFree(c->slab);
+    (void)hipFree(c->counters);
+    (void)hipFree(c->powf_fix);
+    (void)hipFree(c->eos_dev);
+    (void)hipFree(c->red_buf);
+    if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int vah_set_primitives(vah_ctx *c, const double *e, const double *pl, const double *ut, const double *ux,
+                       const double *uy, const double *un, const double *pi) {
+    if (!c || !e || !ut || !ux || !uy || !un) return fail(VAH_ERR_ARG, "vah_set_primitives: NULL argument");
+    HIP_TRY(hipSetDevice(c->device));
+    const long n = (long)c->lat.nx * c->lat.ny * c->lat.nz;
+    const double t = c->hydro.t0;
+    std::vector<double> ptab;
+    if (c->opt.eos == VAH_EOS_TABLE) {
+        ptab.resize(c->eos.n);
+        HIP_TRY(hipMemcpy(ptab.data(), c->eos_dev, sizeof(double) * c->eos.n, hipMemcpyDeviceToHost));
+    }
+    std::vector<double> q((size_t)NQ * n), ev(n), pv(n), u((size_t)4 * n);
+    for (long m = 0; m < n; ++m) {
+        // setConservedVariables (DynamicalVariables.cpp:197-287)
+        double eeq = e[m];
+        double peq = host_eos_p(c, ptab, eeq);
+        double vt = ut[m], vx = ux[m], vy = uy[m], vn = un[m];
+        double pis[10];
+        for (int f = 0; f < 10; ++f) pis[f] = pi ? pi[(size_t)f * n + m] : 0.0;
+        double plv = pl ? pl[m] : 0.0154483 * eeq;
+        double pt = 0.5 * (eeq - plv);
+        double DP = plv - pt;
+        double F = 1 + (vx * vx + vy * vy);
+        double Ltt = DP * t * t * vn * vn / F;
+        double Ltn = DP * vt * vn / F;
+        q[(size_t)TTT * n + m] = (eeq + pt) * vt * vt - pt + Ltt + pis[0];
+        q[(size_t)TTX * n + m] = (eeq + pt) * vt * vx + pis[1];
+        q[(size_t)TTY * n + m] = (eeq + pt) * vt * vy + pis[2];
+        q[(size_t)TTN * n + m] = (eeq + pt) * vt * vn + Ltn + pis[3];
+        q[(size_t)PL * n + m] = plv;
+        for (int f = 0; f < 10; ++f) q[(size_t)(PITT + f) * n + m] = pis[f];
+        ev[m] = eeq;
+        pv[m] = peq;
+        u[m] = vt;
+        u[(size_t)n + m] = vx;
+        u[(size_t)2 * n + m] = vy;
+        u[(size_t)3 * n + m] = vn;
+    }
+    return vah_set_state(c, 0, q.data(), ev.data(), pv.data(), u.data(), u.data());
+}
+
+int vah_set_state(vah_ctx *c, long long nstep, const double *q, const double *e, const double *p, const double *u,
+                  const double *up) {
+    if (!c || !q || !e || !p || !u || !up) return fail(VAH_ERR_ARG, "vah_set_state: NULL argument");
+    if (nstep < 0) return fail(VAH_ERR_ARG, "vah_set_state: negative step count");
+    HIP_TRY(hipSetDevice(c->device));
+    std::vector<double> buf;
+    int rc;
+    if ((rc = upload_fields(c, c->qset[c->qi], NQ, q, buf)) != VAH_OK) return rc;
+    if ((rc = upload_fields(c, c->ebuf[c->ecur], 1, e, buf)) != VAH_OK) return rc;
+    if ((rc = upload_fields(c, c->pbuf[c->ecur], 1, p, buf)) != VAH_OK) return rc;
+    if ((rc = upload_fields(c, c->useq[c->uc], 4, u, buf)) != VAH_OK) return rc;
+    if ((rc = upload_fields(c, c->useq[c->up], 4, up, buf)) != VAH_OK) return rc;
+    if ((rc = fill_ghosts(c)) != VAH_OK) return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->nstep = nstep;
+    c->have_state = true;
+    return VAH_OK;
+}
+
+int vah_step(vah_ctx *c, int nsteps) {
+    if (!c) return fail(VAH_ERR_ARG, "vah_step: NULL context");
+    if (!c->have_state) return fail(VAH_ERR_STATE, "vah_step: no initial state");
+    if (nsteps < 0) return fail(VAH_ERR_ARG, "vah_step: negative step count");
+    HIP_TRY(hipSetDevice(c->device));
+    for (int it = 0; it < nsteps; ++it) {
+        // tau_n = t0 + n dt, as the driver recomputes it (HydroPlugin.cpp:232)
+        const double t = c->hydro.t0 + (double)c->nstep * c->lat.dt;
+        for (int stage = 1; stage <= 2; ++stage) {
+            StageParams P = make_params(c, stage, t);
+            if (c->prof) {
+                hipEvent_t a, b;
+                HIP_TRY(hipEventCreate(&a));
+                HIP_TRY(hipEventCreate(&b));
+                HIP_TRY(hipEventRecord(a, c->stream));
+                launch_stage(c, stage, P);
+                HIP_TRY(hipEventRecord(b, c->stream));
+                c->ev.push_back(a);
+                c->ev.push_back(b);
+                c->ev_kind.push_back(stage - 1);
+            } else {
+                launch_stage(c, stage, P);
+            }
+            HIP_TRY(hipGetLastError());
+        }
+        // rotate roles: q^{n+1} becomes current; (u^n, u^{n-1}) <- (new, old u^n)
+        std::swap(c->qi, c->qo);
+        int old_uc = c->uc;
+        c->uc = c->up;
+        c->up = old_uc;
+        c->nstep++;
+    }
+    return VAH_OK;
+}
+
+int vah_synchronize(vah_ctx *c) {
+    if (!c) return fail(VAH_ERR_ARG, "vah_synchronize: NULL context");
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    unsigned h[4];
+    HIP_TRY(hipMemcpy(h, c->counters, sizeof h, hipMemcpyDeviceToHost));
+    if (h[0] != c->counters_seen[0]) {
+        unsigned d = h[0] - c->counters_seen[0];
+        c->counters_seen[0] = h[0];
+        c->counters_seen[1] = h[1];
+        return fail(VAH_ERR_NAN, "vah_synchronize: %u cell updates produced NaN", d);
+    }
+    c->counters_seen[1] = h[1];
+    return VAH_OK;
+}
+
+int vah_get_state(vah_ctx *c, double *q, double *e, double *p, double *u, double *up) {
+    if (!c) return fail(VAH_ERR_ARG, "vah_get_state: NULL context");
+    HIP_TRY(hipSetDevice(c->device));
+    int rc;
+    if (q && (rc = download_fields(c, c->qset[c->qi], NQ, q)) != VAH_OK) return rc;
+    if (e && (rc = download_fields(c, c->ebuf[c->ecur], 1, e)) != VAH_OK) return rc;
+    if (p && (rc = download_fields(c, c->pbuf[c->ecur], 1, p)) != VAH_OK) return rc;
+    if (u && (rc = download_fields(c, c->useq[c->uc], 4, u)) != VAH_OK) return rc;
+    if (up && (rc = download_fields(c, c->useq[c->up], 4, up)) != VAH_OK) return rc;
+    return VAH_OK;
+}
+
+int vah_get_time(const vah_ctx *c, double *tau, long long *nstep) {
+    if (!c) return fail(VAH_ERR_ARG, "vah_get_time: NULL context");
+    if (tau) *tau = c->hydro.t0 + (double)c->nstep * c->lat.dt;
+    if (nstep) *nstep = c->nstep;
+    return VAH_OK;
+}
+
+int vah_reduce(vah_ctx *c, vah_stats *out) {
+    if (!c || !out) return fail(VAH_ERR_ARG, "vah_reduce: NULL argument");
+    HIP_TRY(hipSetDevice(c->device));
+    hipLaunchKernelGGL(reduce_kernel, dim3(c->red_blocks), dim3(256), 0, c->stream, c->qset[c->qi], c->ebuf[c->ecur],
+                       c->useq[c->uc], c->fs, c->lat.nx, c->lat.ny, c->lat.nz, (long)c->ncx, (long)c->ncx * c->ncy,
+                       c->eos, c->red_buf);
+    HIP_TRY(hipGetLastError());
+    std::vector<double> h((size_t)4 * c->red_blocks);
+    HIP_TRY(hipMemcpyAsync(h.data(), c->red_buf, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    double mx_e = 0, mx_T = 0, sum = 0, mx_v = 0;
+    for (int b = 0; b < c->red_blocks; ++b) {
+        mx_e = std::max(mx_e, h[4 * b]);
+        mx_T = std::max(mx_T, h[4 * b + 1]);
+        sum += h[4 * b + 2];
+        mx_v = std::max(mx_v, h[4 * b + 3]);
+    }
+    unsigned cnt[4];
+    HIP_TRY(hipMemcpy(cnt, c->counters, sizeof cnt, hipMemcpyDeviceToHost));
+    const double tau = c->hydro.t0 + (double)c->nstep * c->lat.dt;
+    out->max_e = mx_e;
+    out->max_T = mx_T;
+    out->total_energy = tau * sum * c->lat.dx * c->lat.dy * c->lat.dz;
+    out->max_vT = mx_v;
+    out->recovery_fallbacks = cnt[1];
+    out->nan_cells = cnt[0];
+    return VAH_OK;
+}
+
+int vah_profile_enable(vah_ctx *c, int on) {
+    if (!c) return fail(VAH_ERR_ARG, "vah_profile_enable: NULL context");
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (auto &e : c->ev) (void)hipEventDestroy(e);
+    c->ev.clear();
+    c->ev_kind.clear();
+    c->prof = on != 0;
+    return VAH_OK;
+}
+
+int vah_profile_read(vah_ctx *c, vah_profile *out) {
+    if (!c || !out) return fail(VAH_ERR_ARG, "vah_profile_read: NULL argument");
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    memset(out, 0, sizeof *out);
+    for (size_t k = 0; k < c->ev_kind.size(); ++k) {
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, c->ev[2 * k], c->ev[2 * k + 1]));
+        out->avg_ms[c->ev_kind[k]] += ms;
+        out->launches[c->ev_kind[k]]++;
+    }
+    for (int s = 0; s < 2; ++s)
+        if (out->launches[s]) out->avg_ms[s] /= out->launches[s];
+    return VAH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,160 @@
+/*
+ * vah.h -- C ABI of the MI355X anisotropic viscous hydrodynamics (VAH) step.
+ *
+ * This is the drop-in boundary for the reference's hot path: the per-step
+ * Kurganov-Tadmor Heun-RK2 update of GPU-VAH (reference paths relative to its
+ * repository root).  Plain C types only; every entry point returns a status
+ * code (VAH_OK == 0) instead of exiting, and vah_last_error() gives the text.
+ *
+ * Evolved fields (15, in this order): T^tautau, T^taux, T^tauy, T^taueta, P_L,
+ * pi^tautau, pi^taux, pi^tauy, pi^taueta, pi^xx, pi^xy, pi^xeta, pi^yy,
+ * pi^yeta, pi^etaeta  (old_cpu-vah_code/include/DynamicalVariables.h:47-75
+ * built with PIMUNU).  Host arrays crossing this interface hold interior
+ * cells only, x fastest: index = i + nx*(j + ny*k); multi-field arrays are
+ * field-major (field f of cell m at [f*n + m], n = nx*ny*nz).
+ */
+#ifndef VAH_H_
+#define VAH_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VAH_NQ 15
+
+enum vah_status {
+    VAH_OK = 0,
+    VAH_ERR_ARG = -1,    /* invalid argument */
+    VAH_ERR_HIP = -2,    /* HIP runtime failure */
+    VAH_ERR_NOMEM = -3,  /* device or host allocation failed */
+    VAH_ERR_NAN = -4,    /* a cell produced NaN (the reference exit(-1)s here) */
+    VAH_ERR_STATE = -5,  /* call out of order (e.g. step before initial state) */
+    VAH_ERR_DEVICE = -6  /* no usable gfx950 device / library built without HIP */
+};
+
+enum vah_field {
+    VAH_TTT = 0, VAH_TTX, VAH_TTY, VAH_TTN, VAH_PL,
+    VAH_PITT, VAH_PITX, VAH_PITY, VAH_PITN, VAH_PIXX, VAH_PIXY, VAH_PIXN, VAH_PIYY, VAH_PIYN, VAH_PINN
+};
+
+/* rhic/include/LatticeParameters.h:17-31 (ghost cells are implicit: 2 per side) */
+typedef struct vah_lattice {
+    int nx, ny, nz;            /* numLatticePointsX / Y / Rapidity */
+    int nt;                    /* numProperTimePoints (the driver runs nt+1 steps) */
+    double dx, dy, dz, dt;     /* latticeSpacingX / Y / Rapidity / ProperTime */
+} vah_lattice;
+
+/* rhic/include/HydroParameters.h:15-21 */
+typedef struct vah_hydro {
+    double t0;                 /* initialProperTimePoint [fm/c] */
+    double etabar;             /* shearViscosityToEntropyDensity */
+    double freezeout_T_GeV;    /* freezeoutTemperatureGeV */
+    int init_pimunu_ns;        /* initializePimunuNavierStokes */
+} vah_hydro;
+
+enum vah_eos_kind {
+    VAH_EOS_CONFORMAL = 0,     /* p = e/3, T = powf(e/13.8997, 1/4) as the reference */
+    VAH_EOS_TABLE = 1          /* tabulated e -> (p, T), linear in ln e */
+};
+
+enum vah_precision {
+    VAH_FP64 = 0,              /* everything in double precision */
+    VAH_FP32_SPLIT = 1         /* stencil, fluxes and sources in float; Newton recovery,
+                                  tau and the state in double */
+};
+
+enum vah_flux_mode {
+    VAH_FLUX_SHARED = 0,       /* each cell face solved once, shared by its two cells */
+    VAH_FLUX_PER_CELL = 1      /* each cell solves both of its faces (the reference's work split) */
+};
+
+/* Tabulated EOS: nodes at ln(e_i) = lne0 + i*h, i = 0..n-1. */
+typedef struct vah_eos_table {
+    int n;
+    double lne0, h;
+    const double *p;           /* pressure at the nodes [fm^-4] */
+    const double *T;           /* temperature at the nodes [fm^-1] */
+} vah_eos_table;
+
+typedef struct vah_options {
+    int eos;                   /* enum vah_eos_kind */
+    vah_eos_table table;       /* read when eos == VAH_EOS_TABLE (copied) */
+    int precision;             /* enum vah_precision */
+    int flux_mode;             /* enum vah_flux_mode */
+    int device;                /* HIP device ordinal */
+    void *stream;              /* hipStream_t to launch on; NULL: a library-owned stream */
+} vah_options;
+
+typedef struct vah_stats {
+    double max_e, max_T;       /* over interior cells [fm^-4], [fm^-1] */
+    double total_energy;       /* tau * sum(T^tautau) dx dy deta  [fm^-3 * fm^3 ...] */
+    double max_vT;             /* max transverse speed |v_T| */
+    long long recovery_fallbacks; /* cells where the Newton recovery fell back (last step) */
+    long long nan_cells;       /* cells that produced NaN (last step) */
+} vah_stats;
+
+typedef struct vah_profile {
+    int launches[2];           /* stage-1 / stage-2 kernel launches recorded */
+    double avg_ms[2];          /* their average duration (HIP events on the launch stream) */
+} vah_profile;
+
+typedef struct vah_ctx vah_ctx;
+
+void vah_default_options(vah_options *opt);
+
+/* Allocates the device state for one lattice: 3 x 15 conserved fields, e and
+ * p (double-buffered), u^mu at three time levels, all with 2 ghost cells per
+ * side.  Replaces allocateDeviceMemory + initializeCUDALaunchParameters +
+ * initializeCUDAConstantParameters (rhic/include/DynamicalVariables.cuh:145-181,
+ * rhic/include/CudaConfiguration.cuh:76-77). */
+int vah_create(const vah_lattice *lat, const vah_hydro *hydro, const vah_options *opt, vah_ctx **out);
+
+/* Initial state from primitives at tau = t0 (setConservedVariables,
+ * old_cpu-vah_code/src/DynamicalVariables.cpp:197-287, then setGhostCells).
+ * pl == NULL: P_L = 0.0154483 e as the reference hardcodes; pi == NULL: pi = 0,
+ * else 10 field-major arrays pitt..pinn.  p = p(e) from the EOS. */
+int vah_set_primitives(vah_ctx *ctx, const double *e, const double *pl, const double *ut, const double *ux,
+                       const double *uy, const double *un, const double *pi);
+
+/* Exact state load: q (15 field-major), e, p, u and up (4 field-major each:
+ * u^tau, u^x, u^y, u^eta); tau = t0 + nstep*dt.  Ghosts are filled on device. */
+int vah_set_state(vah_ctx *ctx, long long nstep, const double *q, const double *e, const double *p, const double *u,
+                  const double *up);
+
+/* Advance nsteps Heun-RK2 steps (twoStepRungeKutta + setCurrentConservedVariables,
+ * rhic/src/FullyDiscreteKurganovTadmorScheme.cu:121-157, rhic/src/HydroPlugin.cpp:223-232).
+ * Asynchronous on the context's stream. */
+int vah_step(vah_ctx *ctx, int nsteps);
+
+/* Blocks until the context's stream is idle; reports NaN cells as VAH_ERR_NAN. */
+int vah_synchronize(vah_ctx *ctx);
+
+/* Copies the current state to the host (copyDeviceToHostMemory).  Any pointer
+ * may be NULL.  Same layouts as vah_set_state. */
+int vah_get_state(vah_ctx *ctx, double *q, double *e, double *p, double *u, double *up);
+
+int vah_get_time(const vah_ctx *ctx, double *tau, long long *nstep);
+
+/* Device-side reductions over interior cells. */
+int vah_reduce(vah_ctx *ctx, vah_stats *out);
+
+/* Per-launch HIP-event timing of the two stage kernels (off by default). */
+int vah_profile_enable(vah_ctx *ctx, int on);
+int vah_profile_read(vah_ctx *ctx, vah_profile *out);
+
+void vah_destroy(vah_ctx *ctx);
+
+/* Text of the last error on this thread. */
+const char *vah_last_error(void);
+
+/* Build identification: "gfx950 <precision modes> <compiler>". */
+const char *vah_build_info(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VAH_H_ */

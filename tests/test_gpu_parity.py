@@ -1,0 +1,82 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle on the
+same seeded inputs, FP64 bar 1e-10 per cell (metric in tests/parity.py)."""
+import numpy as np
+import pytest
+
+from tests.parity import TOL_FP64, assert_parity, gaussian_ic, rest_frame, vah
+
+pytestmark = pytest.mark.gpu
+
+
+def run_pair(oracle_mod, nx, ny, nz, steps, e, u=None, pl=None, pi=None, dx=0.1, dz=0.1, dt=0.01, etabar=0.2,
+             t0=0.5, flux_mode=0, eos_table=None):
+    V = vah()
+    n = nx * ny * nz
+    ut, ux, uy, un = rest_frame(n) if u is None else u
+    g = V.Vah(nx, ny, nz, dx, dx, dz, dt, etabar, t0, flux_mode=flux_mode, eos_table=eos_table)
+    o = oracle_mod.Oracle(nx, ny, nz, dx, dx, dz, dt, etabar, t0, eos_table=eos_table)
+    g.set_primitives(e, ut, ux, uy, un, pl=pl, pi=pi)
+    o.set_primitives(e, ut, ux, uy, un, pl=pl, pi=pi)
+    g.step(steps)
+    g.synchronize()
+    o.step(steps)
+    assert abs(g.time - o.time) == 0.0
+    a, b = g.state(), o.state()
+    g.close()
+    o.close()
+    return a, b, o.time
+
+
+@pytest.mark.parametrize("flux_mode", [0, 1], ids=["shared", "per_cell"])
+def test_c1_shape_200_steps(oracle_mod, flux_mode):
+    """81x81x1, dx 0.1, tau0 0.5, dtau 0.01, 200 steps, conformal, eta/s 0.2, P_L = e/3."""
+    e = gaussian_ic(81, 81)
+    a, b, tau = run_pair(oracle_mod, 81, 81, 1, 200, e, pl=e / 3, flux_mode=flux_mode)
+    worst = assert_parity(a, b, tau, label="C1 shape")
+    print("worst field", worst)
+
+
+def test_reference_pl_init_and_lumps(oracle_mod):
+    """P_L = 0.0154483 e (the reference's hard-coded init), lumpy seeded IC, 60 steps."""
+    e = gaussian_ic(48, 40, seed=5, tilt=0.3)
+    a, b, tau = run_pair(oracle_mod, 48, 40, 1, 60, e)
+    assert_parity(a, b, tau, label="lumpy")
+
+
+def test_3d_small(oracle_mod):
+    """3+1D 24x20x14 with an eta plateau, 25 steps."""
+    e = gaussian_ic(24, 20, 14, dz=0.3, seed=9)
+    a, b, tau = run_pair(oracle_mod, 24, 20, 14, 25, e, pl=e / 3, dz=0.3, dt=0.01)
+    assert_parity(a, b, tau, label="3D")
+
+
+def test_flow_and_shear_initial_state(oracle_mod):
+    """Initial flow and pi from an evolved oracle state, then both continue."""
+    nx, ny, nz = 40, 36, 1
+    e = gaussian_ic(nx, ny, seed=3)
+    n = nx * ny
+    o = oracle_mod.Oracle(nx, ny, nz, 0.1, 0.1, 0.1, 0.01, 0.2, 0.5)
+    o.set_primitives(e, *rest_frame(n), pl=e / 3)
+    o.step(80)
+    st = o.state()
+    tau0 = o.time
+    V = vah()
+    g = V.Vah(nx, ny, nz, 0.1, 0.1, 0.1, 0.01, 0.2, 0.5)
+    g.set_state(80, st["q"], st["e"], st["p"], st["u"], st["up"])
+    assert g.time == tau0
+    g.step(40)
+    g.synchronize()
+    o.step(40)
+    assert_parity(g.state(), o.state(), o.time, label="flow")
+    assert np.max(np.abs(st["u"][1])) > 1e-3  # the continued state really flows
+
+
+def test_ghost_cells_match_neighbours(oracle_mod):
+    """Zero-gradient ghosts: a state uniform in y evolves identically in every row."""
+    nx, ny = 30, 9
+    x = (np.arange(nx) - (nx - 1) / 2) * 0.1
+    e = np.tile(50.0 * np.exp(-x ** 2 / 2.0) + 1e-3, ny)
+    a, b, tau = run_pair(oracle_mod, nx, ny, 1, 30, e, pl=e / 3)
+    E = a["e"].reshape(ny, nx)
+    assert np.all(E == E[0])  # no y structure may appear
+    assert_parity(a, b, tau, label="ghosts")

@@ -69,10 +69,29 @@ class VahError(RuntimeError):
 
 _lib = None
 
-# every entry point include/vah.h declares
-EXPORTS = ("vah_default_options", "vah_create", "vah_set_primitives", "vah_set_state", "vah_step",
-           "vah_synchronize", "vah_get_state", "vah_get_time", "vah_reduce", "vah_profile_enable",
-           "vah_profile_read", "vah_destroy", "vah_last_error", "vah_build_info")
+class Glauber(ctypes.Structure):
+    """vah_glauber of include/vah_host.h."""
+    _fields_ = [("A", ctypes.c_int), ("R", ctypes.c_double), ("a", ctypes.c_double),
+                ("sigma_nn_mb", ctypes.c_double), ("b", ctypes.c_double), ("alpha", ctypes.c_double),
+                ("e_peak", ctypes.c_double), ("nucleon_width", ctypes.c_double), ("seed", ctypes.c_ulonglong),
+                ("eta_flat", ctypes.c_double), ("eta_sigma", ctypes.c_double), ("e_floor", ctypes.c_double)]
+
+
+class Config(ctypes.Structure):
+    """vah_config of include/vah_host.h (the reference's three property files)."""
+    _fields_ = [(n, ctypes.c_int) for n in ("numLatticePointsX", "numLatticePointsY", "numLatticePointsRapidity",
+                                            "numProperTimePoints")] + \
+               [(n, ctypes.c_double) for n in ("latticeSpacingX", "latticeSpacingY", "latticeSpacingRapidity",
+                                               "latticeSpacingProperTime")] + \
+               [(n, ctypes.c_int) for n in ("initialConditionType", "numberOfNucleonsPerNuclei")] + \
+               [(n, ctypes.c_double) for n in ("initialEnergyDensity", "scatteringCrossSectionNN",
+                                               "impactParameter", "fractionOfBinaryCollisions",
+                                               "rapidityVariance", "rapidityMean", "initialProperTimePoint",
+                                               "shearViscosityToEntropyDensity", "freezeoutTemperatureGeV")] + \
+               [("initializePimunuNavierStokes", ctypes.c_int)]
+
+
+HBARC = 0.197326938
 
 
 def build():
@@ -102,6 +121,17 @@ def lib():
         L.vah_destroy.restype = None
         L.vah_last_error.restype = ctypes.c_char_p
         L.vah_build_info.restype = ctypes.c_char_p
+        L.vah_eos_lqcd_table.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp, _dp]
+        L.vah_eos_lqcd_eval.argtypes = [ctypes.c_double, _dp, _dp]
+        L.vah_eos_lqcd_eval.restype = None
+        L.vah_ic_optical_glauber.argtypes = [ctypes.POINTER(Lattice), ctypes.POINTER(Glauber), _dp]
+        L.vah_ic_mc_glauber.argtypes = [ctypes.POINTER(Lattice), ctypes.POINTER(Glauber), _dp,
+                                        ctypes.POINTER(ctypes.c_int)]
+        L.vah_ic_eta_profile.argtypes = [ctypes.c_double] * 3
+        L.vah_ic_eta_profile.restype = ctypes.c_double
+        L.vah_config_load.argtypes = [ctypes.c_char_p, ctypes.POINTER(Config)]
+        L.vah_ic_reference.argtypes = [ctypes.POINTER(Config)] + [_dp] * 5
+        L.vah_run.argtypes = [ctypes.POINTER(Config), ctypes.c_char_p, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -116,6 +146,58 @@ def _arr(a):
         return None, None
     a = np.ascontiguousarray(a, dtype=np.float64).ravel()
     return a, a.ctypes.data_as(_dp)
+
+
+def eos_lqcd_table(n=10000, e_min=1e-8, e_max=1e4):
+    """(lne0, h, p, T) of the crossover EOS tabulated at n nodes uniform in ln e."""
+    p, T = np.empty(n), np.empty(n)
+    lne0, h = ctypes.c_double(), ctypes.c_double()
+    _check(lib().vah_eos_lqcd_table(n, e_min, e_max, p.ctypes.data_as(_dp), T.ctypes.data_as(_dp),
+                                    ctypes.byref(lne0), ctypes.byref(h)))
+    return lne0.value, h.value, p, T
+
+
+def eos_lqcd(T):
+    e, p = ctypes.c_double(), ctypes.c_double()
+    lib().vah_eos_lqcd_eval(T, ctypes.byref(e), ctypes.byref(p))
+    return e.value, p.value
+
+
+def glauber(**kw):
+    """A vah_glauber with the 200 GeV Au+Au optical defaults, overridden by kw."""
+    g = Glauber(A=197, R=6.38, a=0.535, sigma_nn_mb=42.0, b=7.0, alpha=0.5, e_peak=30.0 / HBARC,
+                nucleon_width=0.5, seed=1, eta_flat=1.5, eta_sigma=1.4, e_floor=1e-3)
+    for k, v in kw.items():
+        setattr(g, k, v)
+    return g
+
+
+def ic_optical_glauber(nx, ny, nz, dx, dz, g):
+    lat = Lattice(nx, ny, nz, 0, dx, dx, dz, 0.01)
+    e = np.empty(nx * ny * nz)
+    _check(lib().vah_ic_optical_glauber(ctypes.byref(lat), ctypes.byref(g), e.ctypes.data_as(_dp)))
+    return e
+
+
+def ic_mc_glauber(nx, ny, nz, dx, dz, g):
+    lat = Lattice(nx, ny, nz, 0, dx, dx, dz, 0.01)
+    e = np.empty(nx * ny * nz)
+    nw = ctypes.c_int()
+    _check(lib().vah_ic_mc_glauber(ctypes.byref(lat), ctypes.byref(g), e.ctypes.data_as(_dp), ctypes.byref(nw)))
+    return e, nw.value
+
+
+def config_load(directory):
+    c = Config()
+    _check(lib().vah_config_load(directory.encode(), ctypes.byref(c)))
+    return c
+
+
+def ic_reference(cfg):
+    n = cfg.numLatticePointsX * cfg.numLatticePointsY * cfg.numLatticePointsRapidity
+    arrs = [np.empty(n) for _ in range(5)]
+    _check(lib().vah_ic_reference(ctypes.byref(cfg), *[a.ctypes.data_as(_dp) for a in arrs]))
+    return arrs
 
 
 class Vah:

@@ -142,6 +142,7 @@ StageParams make_params(const vah_ctx *c, int stage, double t) {
     P.dt = c->lat.dt;
     P.dt_inv = 1.0 / c->lat.dt;
     P.etabar = c->hydro.etabar;
+    P.etabar_inv = 1.0 / c->hydro.etabar;
     P.inv_dx = 1.0 / c->lat.dx;
     P.inv_dy = 1.0 / c->lat.dy;
     P.inv_dz = 1.0 / c->lat.dz;

@@ -17,7 +17,7 @@ namespace vah {
 // Everything the source term reads around one cell.  Derivatives are the
 // reference's centred differences: (f[s+1] - f[s-1]) * (1/d)/2.
 struct SrcIn {
-    double t, dt_inv, etabar;
+    double t, dt_inv, etabar_inv;
     double e, p, ut, ux, uy, un;      // at the cell
     double utp, uxp, uyp, unp;        // previous time level at the cell
     double dxut, dxux, dxuy, dxun, dyut, dyux, dyuy, dyun, dnut, dnux, dnuy, dnun;
@@ -37,8 +37,8 @@ __device__ __forceinline__ void source_terms(const Eos &E, const SrcIn &in, cons
     const double dyut = in.dyut, dyux = in.dyux, dyuy = in.dyuy, dyun = in.dyun;
     const double dnut = in.dnut, dnux = in.dnux, dnuy = in.dnuy, dnun = in.dnun;
 
-    const double iut = 1.0 / ut;
-    const double it = 1.0 / t;
+    const double iut = rcp(ut);
+    const double it = rcp(t);
     const double vx = ux * iut, vy = uy * iut, vn = un * iut;
     const double dkvk = ((dxux - vx * dxut) + (dyuy - vy * dyut) + (dnun - vn * dnut)) * iut;
 
@@ -80,9 +80,9 @@ __device__ __forceinline__ void source_terms(const Eos &E, const SrcIn &in, cons
     const double uTduT = ut * uTdtuT + ux * uTdxuT + uy * uTdyuT + un * uTdnuT;
 
     const double F = 1.0 + ux * ux + uy * uy;
-    const double iF = 1.0 / F;
+    const double iF = rcp(F);
     const double iF2 = iF * iF;
-    const double isF = 1.0 / __builtin_sqrt(F);
+    const double isF = rsqrt(F);
 
     const double z0 = t * un * isF;
     const double z3 = ut * it * isF;
@@ -117,7 +117,7 @@ __device__ __forceinline__ void source_terms(const Eos &E, const SrcIn &in, cons
     // anisotropic functions of a = P_L / e
     if (e == 0.0) e = 1.e-7;
     const double T = eos_T(E, e);
-    const double ie = 1.0 / e;
+    const double ie = rcp(e);
     const double a = pl * ie;
     const Powers w = powers_of(a);
     const Fits fr = fits(w);
@@ -160,7 +160,7 @@ __device__ __forceinline__ void source_terms(const Eos &E, const SrcIn &in, cons
     const double tau_pipi = 4.0 * delta_pipi * kThird;
     const double lambda_pipi = fr.Rgamma - 1.0;
 
-    const double taupiInv = 0.2 * T / in.etabar;
+    const double taupiInv = 0.2 * T * in.etabar_inv;
 
     const double psT = pitt * sttT - 2.0 * pitx * stxT - 2.0 * pity * styT + pixx * sxxT + 2.0 * pixy * sxyT +
                        piyy * syyT - 2.0 * pitn * stnT * t2 + 2.0 * pixn * sxnT * t2 + 2.0 * piyn * synT * t2 +

@@ -47,7 +47,7 @@ struct StageParams {
     int tiles_x, tiles_y, zchunks, kz;
     int nblocks;          // real workgroups (the grid is padded to a multiple of 8)
     double t_src, t_rec;  // tau of the sources/fluxes, tau of the recovery
-    double dt, dt_inv, etabar;
+    double dt, dt_inv, etabar, etabar_inv;
     double inv_dx, inv_dy, inv_dz;  // 1/d
     double hx, hy, hz;              // (1/d)/2, the reference's centred-difference factor
     Eos eos;
@@ -78,10 +78,12 @@ __device__ __forceinline__ void face_flux(const Eos &E, double t, double uT_gues
     nan = nan || prim_nan(R) || prim_nan(L);
     double uiR = DIR == 1 ? R.ux : DIR == 2 ? R.uy : R.un;
     double uiL = DIR == 1 ? L.ux : DIR == 2 ? L.uy : L.un;
-    double vR = uiR / R.ut, vL = uiL / L.ut;
+    double vR = uiR * rcp(R.ut), vL = uiL * rcp(L.ut);
     double a = fmax(fabs(vL), fabs(vR));
+    // (F(qR) + F(qL) - a (qR - qL)) / 2 with F(q) = v q
+    double cR = 0.5 * (vR - a), cL = 0.5 * (vL + a);
 #pragma unroll
-    for (int n = 0; n < NQ; ++n) H[n] = 0.5 * (vR * qR[n] + vL * qL[n] - a * (qR[n] - qL[n]));
+    for (int n = 0; n < NQ; ++n) H[n] = cR * qR[n] + cL * qL[n];
 }
 
 __device__ __forceinline__ double uT_of(const double *__restrict__ u, long fs, long s) {
@@ -135,7 +137,7 @@ __device__ __forceinline__ void load_src_in(const StageParams &P, long s, SrcIn 
     const long fs = P.fs, sy = P.sy, sz = P.sz;
     in.t = P.t_src;
     in.dt_inv = P.dt_inv;
-    in.etabar = P.etabar;
+    in.etabar_inv = P.etabar_inv;
     in.e = P.e_in[s];
     in.p = P.p_in[s];
     in.ut = u[s];
@@ -229,7 +231,7 @@ __device__ __forceinline__ void count_flags(const StageParams &P, bool nan, bool
 }
 
 template <bool D3, bool STAGE2, bool SHARED>
-__global__ __launch_bounds__(NTHREADS) void stage_kernel(StageParams P) {
+__global__ __launch_bounds__(NTHREADS, 2) void stage_kernel(StageParams P) {
     __shared__ double Hs[SHARED ? NQ : 1][SHARED ? NTHREADS : 1];
 
     const int b = (int)blockIdx.x;
@@ -296,7 +298,7 @@ __global__ __launch_bounds__(NTHREADS) void stage_kernel(StageParams P) {
                     Hp[n] = Hs[n][fm + 1];
                 }
                 AxisPi ap = axis_pi<1>(P.qin, fs, s, 1, P.hx);
-                add_direction<1>(acc, Hm, Hp, ap, P.ucur[fs + s] / P.ucur[s], P.inv_dx, P.dt);
+                add_direction<1>(acc, Hm, Hp, ap, P.ucur[fs + s] * rcp(P.ucur[s]), P.inv_dx, P.dt);
             }
             __syncthreads();
             // ---- y faces: 16 x 16 faces, face f sits between cells (fi, fj-1) and (fi, fj)
@@ -321,7 +323,7 @@ __global__ __launch_bounds__(NTHREADS) void stage_kernel(StageParams P) {
                     Hp[n] = Hs[n][fm + TX];
                 }
                 AxisPi ap = axis_pi<2>(P.qin, fs, s, sy, P.hy);
-                add_direction<2>(acc, Hm, Hp, ap, P.ucur[2 * fs + s] / P.ucur[s], P.inv_dy, P.dt);
+                add_direction<2>(acc, Hm, Hp, ap, P.ucur[2 * fs + s] * rcp(P.ucur[s]), P.inv_dy, P.dt);
             }
             __syncthreads();
             if (D3 && cell_ok) {
@@ -329,7 +331,7 @@ __global__ __launch_bounds__(NTHREADS) void stage_kernel(StageParams P) {
                 double Hzp[NQ];
                 face_flux<3>(E, P.t_src, uT_of(P.ucur, fs, s), P.qin, fs, s + sz, sz, Hzp, nan);
                 AxisPi ap = axis_pi<3>(P.qin, fs, s, sz, P.hz);
-                add_direction<3>(acc, Hzm, Hzp, ap, P.ucur[3 * fs + s] / P.ucur[s], P.inv_dz, P.dt);
+                add_direction<3>(acc, Hzm, Hzp, ap, P.ucur[3 * fs + s] * rcp(P.ucur[s]), P.inv_dz, P.dt);
 #pragma unroll
                 for (int n = 0; n < NQ; ++n) Hzm[n] = Hzp[n];
             }
@@ -341,21 +343,21 @@ __global__ __launch_bounds__(NTHREADS) void stage_kernel(StageParams P) {
                 face_flux<1>(E, P.t_src, uTs, P.qin, fs, s + 1, 1, Hp, nan);
                 face_flux<1>(E, P.t_src, uTs, P.qin, fs, s, 1, Hm, nan);
                 AxisPi ap = axis_pi<1>(P.qin, fs, s, 1, P.hx);
-                add_direction<1>(acc, Hm, Hp, ap, P.ucur[fs + s] / P.ucur[s], P.inv_dx, P.dt);
+                add_direction<1>(acc, Hm, Hp, ap, P.ucur[fs + s] * rcp(P.ucur[s]), P.inv_dx, P.dt);
             }
             {
                 double Hm[NQ], Hp[NQ];
                 face_flux<2>(E, P.t_src, uTs, P.qin, fs, s + sy, sy, Hp, nan);
                 face_flux<2>(E, P.t_src, uTs, P.qin, fs, s, sy, Hm, nan);
                 AxisPi ap = axis_pi<2>(P.qin, fs, s, sy, P.hy);
-                add_direction<2>(acc, Hm, Hp, ap, P.ucur[2 * fs + s] / P.ucur[s], P.inv_dy, P.dt);
+                add_direction<2>(acc, Hm, Hp, ap, P.ucur[2 * fs + s] * rcp(P.ucur[s]), P.inv_dy, P.dt);
             }
             if (D3) {
                 double Hm[NQ], Hp[NQ];
                 face_flux<3>(E, P.t_src, uTs, P.qin, fs, s + sz, sz, Hp, nan);
                 face_flux<3>(E, P.t_src, uTs, P.qin, fs, s, sz, Hm, nan);
                 AxisPi ap = axis_pi<3>(P.qin, fs, s, sz, P.hz);
-                add_direction<3>(acc, Hm, Hp, ap, P.ucur[3 * fs + s] / P.ucur[s], P.inv_dz, P.dt);
+                add_direction<3>(acc, Hm, Hp, ap, P.ucur[3 * fs + s] * rcp(P.ucur[s]), P.inv_dz, P.dt);
             }
         }
 

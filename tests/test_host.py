@@ -1,0 +1,138 @@
+"""Host-side pieces of the library (no GPU calls): exported symbols, the
+crossover EOS table, initial conditions and the reference's configuration
+loader."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from tests.parity import vah
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for h in ("vah.h", "vah_host.h"):
+        txt = open(os.path.join(ROOT, "include", h)).read()
+        txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+        names |= set(re.findall(r"\b(vah_[a-z0-9_]+)\s*\(", txt))
+    return sorted(names)
+
+
+def test_library_exports_every_declared_entry_point():
+    L = vah().lib()
+    names = declared_functions()
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_eos_table_thermodynamics():
+    V = vah()
+    lne0, h, p, T = V.eos_lqcd_table(10000, 1e-8, 1e4)
+    e = np.exp(lne0 + h * np.arange(len(p)))
+    assert np.all(np.diff(T) > 0) and np.all(np.diff(p) > 0)
+    # nodes reproduce the analytic EOS: e(T_i) = e_i
+    for i in (0, 1234, 5000, 9999):
+        ei, pi = V.eos_lqcd(T[i])
+        assert ei == pytest.approx(e[i], rel=1e-12) and pi == pytest.approx(p[i], rel=1e-12)
+    # s = dp/dT = (e + p)/T along the table (finite differences)
+    dpdT = np.gradient(p, T)
+    s = (e + p) / T
+    assert np.max(np.abs(dpdT[5:-5] / s[5:-5] - 1)) < 2e-4
+    # high-T limit is the reference's conformal EOS e = 13.8997 T^4, p = e/3
+    eh, ph = V.eos_lqcd(5.0)
+    assert eh == pytest.approx(13.8997 * 5.0 ** 4, rel=1e-4) and ph == pytest.approx(eh / 3, rel=1e-4)
+    # crossover: p/T^4 rises through T ~ 155 MeV
+    Tc = 0.155 / V.HBARC
+    lo, hi = V.eos_lqcd(0.8 * Tc), V.eos_lqcd(1.2 * Tc)
+    assert lo[1] / (0.8 * Tc) ** 4 < 0.5 * hi[1] / (1.2 * Tc) ** 4
+
+
+def test_optical_glauber_geometry():
+    V = vah()
+    g = V.glauber()
+    e = V.ic_optical_glauber(81, 81, 1, 0.1, 0.1, g).reshape(81, 81)
+    assert e.max() == pytest.approx(g.e_peak + g.e_floor, rel=1e-12)
+    np.testing.assert_allclose(e, e[:, ::-1], rtol=1e-12)
+    np.testing.assert_allclose(e, e[::-1, :], rtol=1e-12)
+    # b = 7 along x: the overlap is longer in y than in x
+    c = e[40, 40]
+    half_x = np.argmax(e[40, 40:] < c / 2)
+    half_y = np.argmax(e[40:, 40] < c / 2)
+    assert half_y > half_x
+    # central collision, tiny e_floor: rotationally symmetric profile
+    g0 = V.glauber(b=0.0, e_floor=0.0)
+    e0 = V.ic_optical_glauber(41, 41, 1, 0.2, 0.1, g0).reshape(41, 41)
+    np.testing.assert_allclose(e0, e0.T, rtol=1e-12)
+
+
+def test_mc_glauber_is_seeded_and_plausible():
+    V = vah()
+    g = V.glauber(A=208, R=6.62, a=0.546, sigma_nn_mb=70.0, b=6.0, seed=42, e_peak=60 / V.HBARC)
+    e1, n1 = V.ic_mc_glauber(61, 61, 1, 0.2, 0.1, g)
+    e2, n2 = V.ic_mc_glauber(61, 61, 1, 0.2, 0.1, g)
+    assert n1 == n2 and np.array_equal(e1, e2)
+    assert 200 < n1 < 360          # Pb+Pb at b = 6 fm
+    g.seed = 43
+    e3, _ = V.ic_mc_glauber(61, 61, 1, 0.2, 0.1, g)
+    assert not np.array_equal(e1, e3)
+
+
+def test_eta_profile():
+    V = vah()
+    L = V.lib()
+    assert L.vah_ic_eta_profile(0.0, 1.5, 1.4) == 1.0
+    assert L.vah_ic_eta_profile(1.5, 1.5, 1.4) == 1.0
+    assert L.vah_ic_eta_profile(-2.9, 1.5, 1.4) == pytest.approx(np.exp(-1.4 ** 2 / (2 * 1.4 ** 2)))
+
+
+def test_config_defaults_and_reference_files(tmp_path):
+    V = vah()
+    c = V.config_load(str(tmp_path / "missing"))
+    # LatticeParameters.cpp:30-39, InitialConditionParameters.cpp:86-94, HydroParameters.cpp:130-134
+    assert (c.numLatticePointsX, c.numLatticePointsY, c.numLatticePointsRapidity, c.numProperTimePoints) == \
+        (128, 128, 64, 10)
+    assert (c.latticeSpacingX, c.latticeSpacingRapidity, c.latticeSpacingProperTime) == (0.08, 0.3, 0.01)
+    assert (c.initialConditionType, c.numberOfNucleonsPerNuclei) == (2, 208)
+    assert (c.initialProperTimePoint, c.shearViscosityToEntropyDensity, c.freezeoutTemperatureGeV) == \
+        (0.1, 0.0795775, 0.155)
+    assert c.initializePimunuNavierStokes == 1
+    conf = os.path.join(ROOT, "tests", "golden", "rhic-conf")
+    c = V.config_load(conf)
+    assert (c.numLatticePointsX, c.numLatticePointsY, c.numLatticePointsRapidity) == (201, 1, 1)
+    assert c.numProperTimePoints == 10000 and c.latticeSpacingProperTime == 0.005
+    assert c.initialConditionType == 4 and c.initialEnergyDensity == 15.6269
+    assert c.initialProperTimePoint == 0.25 and c.freezeoutTemperatureGeV == 0.140
+    assert c.initializePimunuNavierStokes == 0
+
+
+def test_config_int_literal_for_float_keeps_default(tmp_path):
+    """libconfig does not convert an integer literal for a float lookup, so
+    the reference keeps the default (Properties.cpp:17-22)."""
+    V = vah()
+    (tmp_path / "lattice.properties").write_text("latticeSpacingX = 1\nlatticeSpacingY = 0.25;\n"
+                                                 "numLatticePointsX = 33 // comment\n# numLatticePointsY = 7\n")
+    c = V.config_load(str(tmp_path))
+    assert c.latticeSpacingX == 0.08 and c.latticeSpacingY == 0.25
+    assert c.numLatticePointsX == 33 and c.numLatticePointsY == 128
+
+
+def test_reference_initial_conditions(tmp_path):
+    V = vah()
+    conf = os.path.join(ROOT, "tests", "golden", "rhic-conf")
+    c = V.config_load(conf)
+    c.numLatticePointsX, c.numLatticePointsY = 41, 41
+    e, ut, ux, uy, un = V.ic_reference(c)
+    assert np.all(ut == 1.0) and np.all(ux == 0.0)
+    assert e.min() >= 1e-3 and e.max() > 1.0
+    e2 = V.ic_reference(c)[0]
+    assert np.array_equal(e, e2)  # srand(1328398221) makes it repeatable
+    c.initialConditionType = 0
+    e0 = V.ic_reference(c)[0]
+    assert np.all(e0 == e0[0]) and e0[0] == pytest.approx(13.8997 * 3.05 ** 4, rel=1e-6)
+    c.initialConditionType = 7
+    with pytest.raises(V.VahError):
+        V.ic_reference(c)

@@ -1,0 +1,154 @@
+"""Pins of the oracle's functions that the reference build cannot reach
+(EnergyMomentumTensor.cpp and FullyDiscreteKurganovTadmorScheme.cpp need the
+absent libconfig.h): closed forms, round trips, special cases and invariants.
+"""
+import ctypes
+import ctypes.util
+
+import numpy as np
+import pytest
+
+_dp = ctypes.POINTER(ctypes.c_double)
+libm = ctypes.CDLL(ctypes.util.find_library("m"))
+libm.powf.argtypes = [ctypes.c_float, ctypes.c_float]
+libm.powf.restype = ctypes.c_float
+
+
+def p(a):
+    return a.ctypes.data_as(_dp)
+
+
+def conserved_from_primitives(t, e, pl, ut, ux, uy, un, pi):
+    """T^{tau mu} of the anisotropic decomposition with P_T = (e - P_L)/2:
+    T^{mu nu} = (e + P_T) u^mu u^nu - P_T g^{mu nu} + (P_L - P_T) z^mu z^nu + pi^{mu nu},
+    z^mu = (t u^eta, 0, 0, u^tau / t) / sqrt(1 + u_T^2), metric diag(1, -1, -1, -1/t^2).
+    Built from the tensor definition, independently of the component code."""
+    PT = 0.5 * (e - pl)
+    F = np.sqrt(1 + ux * ux + uy * uy)
+    u = np.array([ut, ux, uy, un])
+    z = np.array([t * un / F, 0.0, 0.0, ut / t / F])
+    g = np.diag([1.0, -1.0, -1.0, -1.0 / t ** 2])
+    T = (e + PT) * np.outer(u, u) - PT * g + (pl - PT) * np.outer(z, z)
+    q = np.zeros(15)
+    q[0:4] = T[0] + pi[0:4]
+    q[4] = pl
+    q[5:15] = pi
+    return q
+
+
+def random_primitives(rng, t):
+    e = 10 ** rng.uniform(-2, 2.5)
+    pl = e * rng.uniform(0.02, 0.9)
+    ux, uy = rng.normal(0, 0.8, 2)
+    un = rng.normal(0, 0.5) / t
+    ut = np.sqrt(1 + ux * ux + uy * uy + t * t * un * un)
+    # a pi orthogonal to u and z is not needed for the inversion: it is subtracted first
+    pi = rng.normal(0, 0.05 * e, 10)
+    return e, pl, ut, ux, uy, un, pi
+
+
+def test_newton_round_trip(oracle_mod):
+    L = oracle_mod.lib()
+    rng = np.random.default_rng(21)
+    out = np.zeros(6)
+    worst = 0.0
+    for _ in range(3000):
+        t = rng.uniform(0.3, 5.0)
+        e, pl, ut, ux, uy, un, pi = random_primitives(rng, t)
+        q = conserved_from_primitives(t, e, pl, ut, ux, uy, un, pi)
+        guess = np.hypot(ux, uy) * rng.uniform(0.8, 1.2)
+        st = L.orc_recover(t, p(q), e, guess, p(out))
+        assert st == 0
+        rel = max(abs(out[0] - e) / e, abs(out[2] - ut) / ut, abs(out[3] - ux) / ut, abs(out[4] - uy) / ut,
+                  abs(out[5] - un) * t / ut)
+        worst = max(worst, rel)
+        assert out[1] == out[0] / 3  # conformal p = e/3
+    assert worst < 1e-11, worst
+
+
+def test_newton_rest_frame_is_exact(oracle_mod):
+    """u = (1,0,0,0): MBT hits the 1e-16 guard, F = 0, and e = T^tautau."""
+    L = oracle_mod.lib()
+    out = np.zeros(6)
+    for e in (1e-3, 1.0, 93.21038818333332):
+        q = np.zeros(15)
+        q[0] = e
+        q[4] = e / 3
+        assert L.orc_recover(0.5, p(q), e, 0.0, p(out)) == 0
+        assert out[0] == e and out[2] == 1.0 and out[3] == 0.0 and out[5] == 0.0
+
+
+def test_newton_fallback(oracle_mod):
+    """Negative energy: the reference falls back to e = 0.999 e_prev, u = (1,0,0,0)
+    (EnergyMomentumTensor.cpp:163-172)."""
+    L = oracle_mod.lib()
+    out = np.zeros(6)
+    q = np.zeros(15)
+    q[0] = -1.0
+    q[1] = 0.3
+    q[4] = 0.1
+    assert L.orc_recover(1.0, p(q), 2.0, 0.1, p(out)) == -1
+    assert out[0] == 2.0 * .999 and out[2] == 1.0 and out[3] == 0.0 and out[4] == 0.0 and out[5] == 0.0
+
+
+def test_slope_closed_form(oracle_mod):
+    L = oracle_mod.lib()
+    rng = np.random.default_rng(4)
+    for _ in range(20000):
+        x, y, z = rng.normal(0, 1, 3)
+        s = L.orc_slope(x, y, z)
+        a, b, c = 1.1 * (y - x), (z - x) / 2, 1.1 * (z - y)
+        if a > 0 and b > 0 and c > 0:
+            assert s == min(a, b, c)
+        elif a < 0 and b < 0 and c < 0:
+            assert s == max(a, b, c)
+        elif (a > 0 and c < 0) or (a < 0 and c > 0):  # local extremum
+            assert s == 0.0
+
+
+def test_conformal_temperature_uses_single_precision_powf(oracle_mod):
+    L = oracle_mod.lib()
+    rng = np.random.default_rng(8)
+    for e in 10 ** rng.uniform(-7, 4, 5000):
+        expect = float(libm.powf(ctypes.c_float(e / 13.8997), ctypes.c_float(0.25)))
+        assert L.orc_temperature_conformal(e) == expect
+
+
+def test_transverse_pressure_conformal_identity(oracle_mod):
+    """For e = 3p the R-bar0 term of P-hat_T drops out: P-hat_T = (e - P_L)/2
+    (AnisotropicDistributionFunctions.cpp:23)."""
+    L = oracle_mod.lib()
+    rng = np.random.default_rng(2)
+    for e in 10 ** rng.uniform(-3, 3, 200):
+        pl = e * rng.uniform(0.01, 0.99)
+        assert L.orc_pt_hat(e, e / 3, pl) == pytest.approx((e - pl) / 2, rel=1e-13)
+
+
+def test_regulator_closed_form(oracle_mod):
+    """pi scaled by tanh(rho)/rho with rho = |pi|/sqrt(e^2 + P_L^2 + 2 P_T^2)
+    when the magnitude ratio dominates (FullyDiscrete...:597-665)."""
+    L = oracle_mod.lib()
+    t = 1.0
+    e, pl = 10.0, 10.0 / 3
+    u = np.array([1.0, 0.0, 0.0, 0.0])
+    for mag in (1e-9, 0.5, 4.0, 40.0):
+        q = np.zeros(15)
+        q[4] = pl
+        q[9] = mag   # pixx
+        q[12] = -mag  # piyy: traceless, orthogonal to u and z
+        ee, pp = np.array([e]), np.array([e / 3])
+        L.orc_regulate_cell(t, p(q), p(ee), p(pp), p(u))
+        ptHat = L.orc_pt_hat(e, e / 3, pl)
+        rho = np.sqrt(2 * mag * mag) / np.sqrt(e * e + pl * pl + 2 * ptHat * ptHat)
+        fac = 1.0 if rho < 1e-7 else np.tanh(rho) / rho
+        assert q[9] == pytest.approx(mag * fac, rel=1e-14)
+        assert q[12] == pytest.approx(-mag * fac, rel=1e-14)
+
+
+def test_regulator_floors(oracle_mod):
+    L = oracle_mod.lib()
+    q = np.zeros(15)
+    q[4] = -1.0
+    ee, pp = np.array([-2.0]), np.array([-1.0])
+    L.orc_regulate_cell(1.0, p(q), p(ee), p(pp), p(np.array([1.0, 0, 0, 0])))
+    assert ee[0] == 1e-7 and pp[0] == 1e-7 and q[4] == 1e-7

@@ -15,7 +15,7 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libvah.so")
+LIB_PATH = os.environ.get("VAH_LIB") or os.path.join(HERE, "libvah.so")
 
 NQ = 15
 FIELDS = ("ttt", "ttx", "tty", "ttn", "pl", "pitt", "pitx", "pity", "pitn",

@@ -151,6 +151,7 @@ StageParams make_params(const vah_ctx *c, int stage, double t) {
     P.hz = 1 / c->lat.dz / 2;
     P.eos = c->eos;
     P.counters = c->counters;
+    P.emax = nullptr;
     const int e_in = stage == 1 ? c->ecur : 1 - c->ecur;
     P.e_in = c->ebuf[e_in];
     P.p_in = c->pbuf[e_in];
@@ -180,19 +181,29 @@ StageParams make_params(const vah_ctx *c, int stage, double t) {
     return P;
 }
 
-template <bool D3, bool S2, bool SH>
+template <bool D3, bool S2, bool SH, typename R>
 void launch(const vah_ctx *c, const StageParams &P) {
-    hipLaunchKernelGGL((stage_kernel<D3, S2, SH>), dim3(c->grid), dim3(NTHREADS), 0, c->stream, P);
+    hipLaunchKernelGGL((stage_kernel<D3, S2, SH, R>), dim3(c->grid), dim3(NTHREADS), 0, c->stream, P);
+}
+
+template <bool D3, bool S2>
+void launch_variant(const vah_ctx *c, const StageParams &P) {
+    if (c->opt.precision == VAH_FP32_SPLIT)
+        launch<D3, S2, true, float>(c, P);
+    else if (c->opt.flux_mode == VAH_FLUX_SHARED)
+        launch<D3, S2, true, double>(c, P);
+    else
+        launch<D3, S2, false, double>(c, P);
 }
 
 void launch_stage(const vah_ctx *c, int stage, const StageParams &P) {
-    const bool d3 = c->lat.nz > 1, sh = c->opt.flux_mode == VAH_FLUX_SHARED;
+    const bool d3 = c->lat.nz > 1;
     if (d3) {
-        if (stage == 1) sh ? launch<true, false, true>(c, P) : launch<true, false, false>(c, P);
-        else sh ? launch<true, true, true>(c, P) : launch<true, true, false>(c, P);
+        if (stage == 1) launch_variant<true, false>(c, P);
+        else launch_variant<true, true>(c, P);
     } else {
-        if (stage == 1) sh ? launch<false, false, true>(c, P) : launch<false, false, false>(c, P);
-        else sh ? launch<false, true, true>(c, P) : launch<false, true, false>(c, P);
+        if (stage == 1) launch_variant<false, false>(c, P);
+        else launch_variant<false, true>(c, P);
     }
 }
 
@@ -300,7 +311,7 @@ void vah_default_options(vah_options *opt) {
 
 const char *vah_last_error(void) { return g_err.c_str(); }
 
-const char *vah_build_info(void) { return "vah-mi355x gfx950 fp64 fused-KT-stage"; }
+const char *vah_build_info(void) { return "vah-mi355x gfx950 fused-KT-stage fp64 fp32-split"; }
 
 int vah_create(const vah_lattice *lat, const vah_hydro *hydro, const vah_options *opt_in, vah_ctx **out) {
     if (!out) return fail(VAH_ERR_ARG, "vah_create: out is NULL");
@@ -313,7 +324,10 @@ int vah_create(const vah_lattice *lat, const vah_hydro *hydro, const vah_options
         opt = *opt_in;
     else
         vah_default_options(&opt);
-    if (opt.precision != VAH_FP64) return fail(VAH_ERR_ARG, "vah_create: precision %d not built", opt.precision);
+    if (opt.precision != VAH_FP64 && opt.precision != VAH_FP32_SPLIT)
+        return fail(VAH_ERR_ARG, "vah_create: unknown precision %d", opt.precision);
+    if (opt.precision == VAH_FP32_SPLIT && opt.flux_mode != VAH_FLUX_SHARED)
+        return fail(VAH_ERR_ARG, "vah_create: the split-precision form uses shared faces only");
     if (opt.flux_mode != VAH_FLUX_SHARED && opt.flux_mode != VAH_FLUX_PER_CELL)
         return fail(VAH_ERR_ARG, "vah_create: unknown flux mode %d", opt.flux_mode);
     if (opt.eos == VAH_EOS_TABLE && (opt.table.n < 2 || !opt.table.p || !opt.table.T || !(opt.table.h > 0)))

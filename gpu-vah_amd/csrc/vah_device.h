@@ -48,6 +48,9 @@ __device__ __forceinline__ double fdiv(double a, double b) {
     return __builtin_fma(e, r, q);
 }
 
+__device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
+
 __device__ __forceinline__ double rsqrt(double x) {
     double y = __builtin_amdgcn_rsq(x);
     double h = 0.5 * x;

@@ -24,219 +24,220 @@ struct SrcIn {
     double dxp, dyp, dnp, dxe, dye, dne, dxpl, dypl, dnpl;
 };
 
-template <typename QA>
-__device__ __forceinline__ void source_terms(const Eos &E, const SrcIn &in, const QA &Q, double (&S)[NQ],
-                                             bool &nan) {
-    const double t = in.t;
-    const double ttt = Q[TTT], ttx = Q[TTX], tty = Q[TTY], ttn = Q[TTN], pl = Q[PL];
-    const double pitt = Q[PITT], pitx = Q[PITX], pity = Q[PITY], pitn = Q[PITN], pixx = Q[PIXX],
+// R is the arithmetic type of the source terms: double, or float for the
+// split-precision form (the rational fits and the temperature stay double).
+template <typename R, typename QA>
+__device__ __forceinline__ void source_terms(const Eos &E, const SrcIn &in, const QA &Q, R (&S)[NQ], bool &nan) {
+    const R t = in.t;
+    const R ttt = Q[TTT], ttx = Q[TTX], tty = Q[TTY], ttn = Q[TTN], pl = Q[PL];
+    const R pitt = Q[PITT], pitx = Q[PITX], pity = Q[PITY], pitn = Q[PITN], pixx = Q[PIXX],
                  pixy = Q[PIXY], pixn = Q[PIXN], piyy = Q[PIYY], piyn = Q[PIYN], pinn = Q[PINN];
-    double e = in.e;
-    const double p = in.p, ut = in.ut, ux = in.ux, uy = in.uy, un = in.un;
-    const double dxut = in.dxut, dxux = in.dxux, dxuy = in.dxuy, dxun = in.dxun;
-    const double dyut = in.dyut, dyux = in.dyux, dyuy = in.dyuy, dyun = in.dyun;
-    const double dnut = in.dnut, dnux = in.dnux, dnuy = in.dnuy, dnun = in.dnun;
+    R e = (R)in.e;
+    const R p = in.p, ut = in.ut, ux = in.ux, uy = in.uy, un = in.un;
+    const R dxut = in.dxut, dxux = in.dxux, dxuy = in.dxuy, dxun = in.dxun;
+    const R dyut = in.dyut, dyux = in.dyux, dyuy = in.dyuy, dyun = in.dyun;
+    const R dnut = in.dnut, dnux = in.dnux, dnuy = in.dnuy, dnun = in.dnun;
 
-    const double iut = rcp(ut);
-    const double it = rcp(t);
-    const double vx = ux * iut, vy = uy * iut, vn = un * iut;
-    const double dkvk = ((dxux - vx * dxut) + (dyuy - vy * dyut) + (dnun - vn * dnut)) * iut;
+    const R iut = rcp(ut);
+    const R it = rcp(t);
+    const R vx = ux * iut, vy = uy * iut, vn = un * iut;
+    const R dkvk = ((dxux - vx * dxut) + (dyuy - vy * dyut) + (dnun - vn * dnut)) * iut;
 
-    const double t2 = t * t, it2 = it * it, un2 = un * un, ut2 = ut * ut;
+    const R t2 = t * t, it2 = it * it, un2 = un * un, ut2 = ut * ut;
 
-    const double dtut = (ut - in.utp) * in.dt_inv, dtux = (ux - in.uxp) * in.dt_inv;
-    const double dtuy = (uy - in.uyp) * in.dt_inv, dtun = (un - in.unp) * in.dt_inv;
+    const R dtut = (ut - in.utp) * in.dt_inv, dtux = (ux - in.uxp) * in.dt_inv;
+    const R dtuy = (uy - in.uyp) * in.dt_inv, dtun = (un - in.unp) * in.dt_inv;
 
     // (u.d) u^mu and covariant derivatives
-    const double dut = ut * dtut + ux * dxut + uy * dyut + un * dnut;
-    const double dux = ut * dtux + ux * dxux + uy * dyux + un * dnux;
-    const double duy = ut * dtuy + ux * dxuy + uy * dyuy + un * dnuy;
-    const double dun = ut * dtun + ux * dxun + uy * dyun + un * dnun;
-    const double Dut = dut + t * un * un;
-    const double Dux = -dux, Duy = -duy;
-    const double Dun = -t2 * (dun + 2.0 * ut * un * it);
+    const R dut = ut * dtut + ux * dxut + uy * dyut + un * dnut;
+    const R dux = ut * dtux + ux * dxux + uy * dyux + un * dnux;
+    const R duy = ut * dtuy + ux * dxuy + uy * dyuy + un * dnuy;
+    const R dun = ut * dtun + ux * dxun + uy * dyun + un * dnun;
+    const R Dut = dut + t * un * un;
+    const R Dux = -dux, Duy = -duy;
+    const R Dun = -t2 * (dun + R(2.0) * ut * un * it);
 
-    const double theta = ut * it + dtut + dxux + dyuy + dnun;
-    const double th3 = theta * kThird;
+    const R theta = ut * it + dtut + dxux + dyuy + dnun;
+    const R th3 = theta * R(kThird);
 
     // shear tensor sigma^{mu nu}
-    const double stt = -t * ut * un2 + (dtut - ut * dut) + (ut2 - 1.0) * th3;
-    const double stx = -0.5 * (t * un2 * ux) + 0.5 * (dtux - dxut) - 0.5 * (ux * dut + ut * dux) + ut * ux * th3;
-    const double sty = -0.5 * (t * un2 * uy) + 0.5 * (dtuy - dyut) - 0.5 * (uy * dut + ut * duy) + ut * uy * th3;
-    const double stn = -un * (2.0 * ut2 + t2 * un2) * 0.5 * it + 0.5 * (dtun - dnut * it2) - 0.5 * (un * dut + ut * dun) +
+    const R stt = -t * ut * un2 + (dtut - ut * dut) + (ut2 - R(1.0)) * th3;
+    const R stx = -R(0.5) * (t * un2 * ux) + R(0.5) * (dtux - dxut) - R(0.5) * (ux * dut + ut * dux) + ut * ux * th3;
+    const R sty = -R(0.5) * (t * un2 * uy) + R(0.5) * (dtuy - dyut) - R(0.5) * (uy * dut + ut * duy) + ut * uy * th3;
+    const R stn = -un * (R(2.0) * ut2 + t2 * un2) * R(0.5) * it + R(0.5) * (dtun - dnut * it2) - R(0.5) * (un * dut + ut * dun) +
                        ut * un * th3;
-    const double sxx = -(dxux + ux * dux) + (1.0 + ux * ux) * th3;
-    const double sxy = -0.5 * (dxuy + dyux) - 0.5 * (uy * dux + ux * duy) + ux * uy * th3;
-    const double sxn = -ut * ux * un * it - 0.5 * (dxun + dnux * it2) - 0.5 * (un * dux + ux * dun) + ux * un * th3;
-    const double syy = -(dyuy + uy * duy) + (1.0 + uy * uy) * th3;
-    const double syn = -ut * uy * un * it - 0.5 * (dyun + dnuy * it2) - 0.5 * (un * duy + uy * dun) + uy * un * th3;
-    const double snn = -ut * (1.0 + 2.0 * t2 * un2) * it2 * it - dnun * it2 - un * dun + (it2 + un2) * th3;
+    const R sxx = -(dxux + ux * dux) + (R(1.0) + ux * ux) * th3;
+    const R sxy = -R(0.5) * (dxuy + dyux) - R(0.5) * (uy * dux + ux * duy) + ux * uy * th3;
+    const R sxn = -ut * ux * un * it - R(0.5) * (dxun + dnux * it2) - R(0.5) * (un * dux + ux * dun) + ux * un * th3;
+    const R syy = -(dyuy + uy * duy) + (R(1.0) + uy * uy) * th3;
+    const R syn = -ut * uy * un * it - R(0.5) * (dyun + dnuy * it2) - R(0.5) * (un * duy + uy * dun) + uy * un * th3;
+    const R snn = -ut * (R(1.0) + R(2.0) * t2 * un2) * it2 * it - dnun * it2 - un * dun + (it2 + un2) * th3;
 
     // longitudinal unit vector z^mu and the transverse / longitudinal split
-    const double uTdtuT = ux * dtux + uy * dtuy;
-    const double uTdxuT = ux * dxux + uy * dxuy;
-    const double uTdyuT = ux * dyux + uy * dyuy;
-    const double uTdnuT = ux * dnux + uy * dnuy;
-    const double uTduT = ut * uTdtuT + ux * uTdxuT + uy * uTdyuT + un * uTdnuT;
+    const R uTdtuT = ux * dtux + uy * dtuy;
+    const R uTdxuT = ux * dxux + uy * dxuy;
+    const R uTdyuT = ux * dyux + uy * dyuy;
+    const R uTdnuT = ux * dnux + uy * dnuy;
+    const R uTduT = ut * uTdtuT + ux * uTdxuT + uy * uTdyuT + un * uTdnuT;
 
-    const double F = 1.0 + ux * ux + uy * uy;
-    const double iF = rcp(F);
-    const double iF2 = iF * iF;
-    const double isF = rsqrt(F);
+    const R F = R(1.0) + ux * ux + uy * uy;
+    const R iF = rcp(F);
+    const R iF2 = iF * iF;
+    const R isF = rsqrt(F);
 
-    const double z0 = t * un * isF;
-    const double z3 = ut * it * isF;
-    const double z02 = z0 * z0, z0z3 = z0 * z3, z32 = z3 * z3;
+    const R z0 = t * un * isF;
+    const R z3 = ut * it * isF;
+    const R z02 = z0 * z0, z0z3 = z0 * z3, z32 = z3 * z3;
 
-    const double dz0 = (ut * un + t * dun - t * un * iF * uTduT) * isF;
-    const double dz3 = (-ut2 * it2 + dut * it - ut * it * iF * uTduT) * isF;
-    const double Dz0 = dz0 + t * un * z3;
-    const double Dz3 = -t2 * (dz3 + (ut * z3 + un * z0) * it);
+    const R dz0 = (ut * un + t * dun - t * un * iF * uTduT) * isF;
+    const R dz3 = (-ut2 * it2 + dut * it - ut * it * iF * uTduT) * isF;
+    const R Dz0 = dz0 + t * un * z3;
+    const R Dz3 = -t2 * (dz3 + (ut * z3 + un * z0) * it);
 
-    const double A = z02 * stt - 2.0 * t2 * z0z3 * stn + t2 * t2 * z32 * snn;
-    const double B0 = z0 * stt - t2 * z3 * stn;
-    const double B1 = z0 * stx - t2 * z3 * sxn;
-    const double B2 = z0 * sty - t2 * z3 * syn;
-    const double B3 = z0 * stn - t2 * z3 * snn;
+    const R A = z02 * stt - R(2.0) * t2 * z0z3 * stn + t2 * t2 * z32 * snn;
+    const R B0 = z0 * stt - t2 * z3 * stn;
+    const R B1 = z0 * stx - t2 * z3 * sxn;
+    const R B2 = z0 * sty - t2 * z3 * syn;
+    const R B3 = z0 * stn - t2 * z3 * snn;
 
-    const double thetaT = 2.0 * th3 + A;
-    const double zDzu = theta - thetaT;
+    const R thetaT = R(2.0) * th3 + A;
+    const R zDzu = theta - thetaT;
 
-    const double hA = 0.5 * A;
-    const double sttT = stt + 2.0 * z0 * B0 + z02 * A - (1.0 - ut * ut + z02) * hA;
-    const double stxT = stx + z0 * B1 + ut * ux * hA;
-    const double styT = sty + z0 * B2 + ut * uy * hA;
-    const double stnT = stn + z0 * B3 + z3 * B0 + z0z3 * A - (-ut * un + z0z3) * hA;
-    const double sxxT = sxx + (1.0 + ux * ux) * hA;
-    const double sxyT = sxy + ux * uy * hA;
-    const double sxnT = sxn + z3 * B1 + ux * un * hA;
-    const double syyT = syy + (1.0 + uy * uy) * hA;
-    const double synT = syn + z3 * B2 + uy * un * hA;
-    const double snnT = snn + 2.0 * z3 * B3 + z32 * A - (-it2 - un * un + z32) * hA;
+    const R hA = R(0.5) * A;
+    const R sttT = stt + R(2.0) * z0 * B0 + z02 * A - (R(1.0) - ut * ut + z02) * hA;
+    const R stxT = stx + z0 * B1 + ut * ux * hA;
+    const R styT = sty + z0 * B2 + ut * uy * hA;
+    const R stnT = stn + z0 * B3 + z3 * B0 + z0z3 * A - (-ut * un + z0z3) * hA;
+    const R sxxT = sxx + (R(1.0) + ux * ux) * hA;
+    const R sxyT = sxy + ux * uy * hA;
+    const R sxnT = sxn + z3 * B1 + ux * un * hA;
+    const R syyT = syy + (R(1.0) + uy * uy) * hA;
+    const R synT = syn + z3 * B2 + uy * un * hA;
+    const R snnT = snn + R(2.0) * z3 * B3 + z32 * A - (-it2 - un * un + z32) * hA;
 
     // anisotropic functions of a = P_L / e
-    if (e == 0.0) e = 1.e-7;
-    const double T = eos_T(E, e);
-    const double ie = rcp(e);
-    const double a = pl * ie;
-    const Powers w = powers_of(a);
-    const Fits fr = fits(w);
+    if (e == R(0.0)) e = R(1.e-7);
+    const R T = (R)eos_T(E, (double)e);
+    const R ie = rcp(e);
+    const R a = pl * ie;
+    const Fits fr = fits(powers_of((double)a));
+    const R Rtilde = (R)fr.Rtilde, Rbar0 = (R)fr.Rbar0, Rgamma = (R)fr.Rgamma, Rbar0P = (R)fr.Rbar0P;
 
-    const double ptHat = 0.5 * ((e - pl) - fr.Rbar0 * (e - 3.0 * p));
-    const double pt = ptHat;
-    const double DP = pl - pt;
+    const R ptHat = R(0.5) * ((e - pl) - Rbar0 * (e - R(3.0) * p));
+    const R pt = ptHat;
+    const R DP = pl - pt;
 
-    const double Ltt = DP * t2 * un2 * iF;
-    const double Ltn = DP * ut * un * iF;
-    const double Lnn = DP * ut2 * iF * it2;
-    const double tnn = (e + pt) * un * un + pt * it2 + Lnn + pinn;
+    const R Ltt = DP * t2 * un2 * iF;
+    const R Ltn = DP * ut * un * iF;
+    const R Lnn = DP * ut2 * iF * it2;
+    const R tnn = (e + pt) * un * un + pt * it2 + Lnn + pinn;
 
-    const double dxpl = in.dxpl, dypl = in.dypl, dnpl = in.dnpl;
-    const double dxe = in.dxe, dye = in.dye, dne = in.dne;
-    const double dxRbar0 = fr.Rbar0P * (dxpl - a * dxe) * ie;
-    const double dyRbar0 = fr.Rbar0P * (dypl - a * dye) * ie;
-    const double dnRbar0 = fr.Rbar0P * (dnpl - a * dne) * ie;
-    const double em3p = e - 3.0 * p;
-    const double dxptHat = 0.5 * (dxe - dxpl - fr.Rbar0 * (dxe - 3.0 * in.dxp) - em3p * dxRbar0);
-    const double dyptHat = 0.5 * (dye - dypl - fr.Rbar0 * (dye - 3.0 * in.dyp) - em3p * dyRbar0);
-    const double dnptHat = 0.5 * (dne - dnpl - fr.Rbar0 * (dne - 3.0 * in.dnp) - em3p * dnRbar0);
+    const R dxpl = (R)in.dxpl, dypl = (R)in.dypl, dnpl = (R)in.dnpl;
+    const R dxe = in.dxe, dye = in.dye, dne = in.dne;
+    const R dxRbar0 = Rbar0P * (dxpl - a * dxe) * ie;
+    const R dyRbar0 = Rbar0P * (dypl - a * dye) * ie;
+    const R dnRbar0 = Rbar0P * (dnpl - a * dne) * ie;
+    const R em3p = e - R(3.0) * p;
+    const R dxptHat = R(0.5) * (dxe - dxpl - Rbar0 * (dxe - R(3.0) * (R)in.dxp) - em3p * dxRbar0);
+    const R dyptHat = R(0.5) * (dye - dypl - Rbar0 * (dye - R(3.0) * (R)in.dyp) - em3p * dyRbar0);
+    const R dnptHat = R(0.5) * (dne - dnpl - Rbar0 * (dne - R(3.0) * (R)in.dnp) - em3p * dnRbar0);
 
-    const double dxDP = dxpl - dxptHat, dyDP = dypl - dyptHat, dnDP = dnpl - dnptHat;
+    const R dxDP = dxpl - dxptHat, dyDP = dypl - dyptHat, dnDP = dnpl - dnptHat;
 
-    const double DPiF = DP * iF, DPiF2 = DP * iF2;
-    const double dxLtt = 2.0 * DPiF * dxun * t2 * un + dxDP * t2 * un2 * iF - 2.0 * DPiF2 * uTdxuT * t2 * un2;
-    const double dyLtt = 2.0 * DPiF * dyun * t2 * un + dyDP * t2 * un2 * iF - 2.0 * DPiF2 * uTdyuT * t2 * un2;
-    const double dnLtt = 2.0 * DPiF * dnun * t2 * un + dnDP * t2 * un2 * iF - 2.0 * DPiF2 * uTdnuT * t2 * un2;
-    const double dxLtn = DPiF * dxut * un + DPiF * dxun * ut + dxDP * un * ut * iF - 2.0 * DPiF2 * uTdxuT * un * ut;
-    const double dyLtn = DPiF * dyut * un + DPiF * dyun * ut + dyDP * un * ut * iF - 2.0 * DPiF2 * uTdyuT * un * ut;
-    const double dnLtn = DPiF * dnut * un + DPiF * dnun * ut + dnDP * un * ut * iF - 2.0 * DPiF2 * uTdnuT * un * ut;
-    const double dnLnn = (2.0 * DPiF * dnut * ut + dnDP * ut2 * iF - 2.0 * DPiF2 * uTdnuT * ut2) * it2;
+    const R DPiF = DP * iF, DPiF2 = DP * iF2;
+    const R dxLtt = R(2.0) * DPiF * dxun * t2 * un + dxDP * t2 * un2 * iF - R(2.0) * DPiF2 * uTdxuT * t2 * un2;
+    const R dyLtt = R(2.0) * DPiF * dyun * t2 * un + dyDP * t2 * un2 * iF - R(2.0) * DPiF2 * uTdyuT * t2 * un2;
+    const R dnLtt = R(2.0) * DPiF * dnun * t2 * un + dnDP * t2 * un2 * iF - R(2.0) * DPiF2 * uTdnuT * t2 * un2;
+    const R dxLtn = DPiF * dxut * un + DPiF * dxun * ut + dxDP * un * ut * iF - R(2.0) * DPiF2 * uTdxuT * un * ut;
+    const R dyLtn = DPiF * dyut * un + DPiF * dyun * ut + dyDP * un * ut * iF - R(2.0) * DPiF2 * uTdyuT * un * ut;
+    const R dnLtn = DPiF * dnut * un + DPiF * dnun * ut + dnDP * un * ut * iF - R(2.0) * DPiF2 * uTdnuT * un * ut;
+    const R dnLnn = (R(2.0) * DPiF * dnut * ut + dnDP * ut2 * iF - R(2.0) * DPiF2 * uTdnuT * ut2) * it2;
 
-    const double zeta_zz = fr.Rtilde * e - 3.0 * pl;
-    const double zeta_zT = 0.5 * (fr.Rtilde * e + pl);
+    const R zeta_zz = Rtilde * e - R(3.0) * pl;
+    const R zeta_zT = R(0.5) * (Rtilde * e + pl);
 
-    const double lambda_lpi = fr.Rgamma;
-    const double delta_pipi = 0.5 * (3.0 + fr.Rgamma);
-    const double tau_pipi = 4.0 * delta_pipi * kThird;
-    const double lambda_pipi = fr.Rgamma - 1.0;
+    const R lambda_lpi = Rgamma;
+    const R delta_pipi = R(0.5) * (R(3.0) + Rgamma);
+    const R tau_pipi = R(4.0) * delta_pipi * R(kThird);
+    const R lambda_pipi = Rgamma - R(1.0);
 
-    const double taupiInv = 0.2 * T * in.etabar_inv;
+    const R taupiInv = R(0.2) * T * in.etabar_inv;
 
-    const double psT = pitt * sttT - 2.0 * pitx * stxT - 2.0 * pity * styT + pixx * sxxT + 2.0 * pixy * sxyT +
-                       piyy * syyT - 2.0 * pitn * stnT * t2 + 2.0 * pixn * sxnT * t2 + 2.0 * piyn * synT * t2 +
+    const R psT = pitt * sttT - R(2.0) * pitx * stxT - R(2.0) * pity * styT + pixx * sxxT + R(2.0) * pixy * sxyT +
+                       piyy * syyT - R(2.0) * pitn * stnT * t2 + R(2.0) * pixn * sxnT * t2 + R(2.0) * piyn * synT * t2 +
                        pinn * snnT * t2 * t2;
 
-    const double dPL = -(pl - p) * taupiInv + zeta_zz * zDzu - zeta_zT * thetaT - lambda_lpi * psT;
+    const R dPL = -(pl - p) * taupiInv + zeta_zz * zDzu - zeta_zT * thetaT - lambda_lpi * psT;
     S[PL] = dPL * iut + dkvk * pl;
 
     S[TTT] = -(ttt * it + t * tnn) + dkvk * (Ltt + pitt - pt) - vx * dxptHat - vy * dyptHat - vn * dnptHat +
              vx * dxLtt + vy * dyLtt + vn * dnLtt - dnLtn;
     S[TTX] = -ttx * it - dxptHat + dkvk * pitx;
     S[TTY] = -tty * it - dyptHat + dkvk * pity;
-    S[TTN] = -3.0 * ttn * it - dnptHat * it2 + dkvk * (Ltn + pitn) + vx * dxLtn + vy * dyLtn + 2.0 * vn * dnLtn -
+    S[TTN] = -R(3.0) * ttn * it - dnptHat * it2 + dkvk * (Ltn + pitn) + vx * dxLtn + vy * dyLtn + R(2.0) * vn * dnLtn -
              dnLnn;
 
     nan = nan || __builtin_isnan(S[PL]) || __builtin_isnan(S[TTT]);
 
     // transverse shear stress pi_perp^{mu nu}
-    const double etaBar_TC = 0.125 * ((3.0 - fr.Rtilde) * e - 2.0 * pl);
+    const R etaBar_TC = R(0.125) * ((R(3.0) - Rtilde) * e - R(2.0) * pl);
 
-    const double piDt = pitt * Dut + pitx * Dux + pity * Duy + pitn * Dun;
-    const double I1tt = 2.0 * ut * piDt - 2.0 * z0 * (pitt * Dz0 + pitn * Dz3);
-    const double I1tx = (pitt * ux + pitx * ut) * Dut + (pitx * ux + pixx * ut) * Dux + (pity * ux + pixy * ut) * Duy +
+    const R piDt = pitt * Dut + pitx * Dux + pity * Duy + pitn * Dun;
+    const R I1tt = R(2.0) * ut * piDt - R(2.0) * z0 * (pitt * Dz0 + pitn * Dz3);
+    const R I1tx = (pitt * ux + pitx * ut) * Dut + (pitx * ux + pixx * ut) * Dux + (pity * ux + pixy * ut) * Duy +
                         (pitn * ux + pixn * ut) * Dun - ((pitx * z0) * Dz0 + (pitn * ux + pixn * z0) * Dz3);
-    const double I1ty = (pitt * uy + pity * ut) * Dut + (pitx * uy + pixy * ut) * Dux + (pity * uy + piyy * ut) * Duy +
+    const R I1ty = (pitt * uy + pity * ut) * Dut + (pitx * uy + pixy * ut) * Dux + (pity * uy + piyy * ut) * Duy +
                         (pitn * uy + piyn * ut) * Dun - ((pity * z0) * Dz0 + (pitn * uy + piyn * z0) * Dz3);
-    const double I1tn = (pitt * un + pitn * ut) * Dut + (pitx * un + pixn * ut) * Dux + (pity * un + piyn * ut) * Duy +
+    const R I1tn = (pitt * un + pitn * ut) * Dut + (pitx * un + pixn * ut) * Dux + (pity * un + piyn * ut) * Duy +
                         (pitn * un + pinn * ut) * Dun -
                         ((pitt * z3 + pitn * z0) * Dz0 + (pitn * z3 + pinn * z0) * Dz3);
-    const double I1xx = 2.0 * ux * (pitx * Dut + pixx * Dux + pixy * Duy + pixn * Dun);
-    const double I1xy = (pitx * uy + pity * ux) * Dut + (pixx * uy + pixy * ux) * Dux + (pixy * uy + piyy * ux) * Duy +
+    const R I1xx = R(2.0) * ux * (pitx * Dut + pixx * Dux + pixy * Duy + pixn * Dun);
+    const R I1xy = (pitx * uy + pity * ux) * Dut + (pixx * uy + pixy * ux) * Dux + (pixy * uy + piyy * ux) * Duy +
                         (pixn * uy + piyn * ux) * Dun;
-    const double I1xn = (pitx * un + pitn * ux) * Dut + (pixx * un + pixn * ux) * Dux + (pixy * un + piyn * ux) * Duy +
+    const R I1xn = (pitx * un + pitn * ux) * Dut + (pixx * un + pixn * ux) * Dux + (pixy * un + piyn * ux) * Duy +
                         (pixn * un + pinn * ux) * Dun - ((pitx * z3) * Dz0 + (pixn * z3) * Dz3);
-    const double I1yy = 2.0 * uy * (pity * Dut + pixy * Dux + piyy * Duy + piyn * Dun);
-    const double I1yn = (pity * un + pitn * uy) * Dut + (pixy * un + pixn * uy) * Dux + (piyy * un + piyn * uy) * Duy +
+    const R I1yy = R(2.0) * uy * (pity * Dut + pixy * Dux + piyy * Duy + piyn * Dun);
+    const R I1yn = (pity * un + pitn * uy) * Dut + (pixy * un + pixn * uy) * Dux + (piyy * un + piyn * uy) * Duy +
                         (piyn * un + pinn * uy) * Dun - ((pity * z3) * Dz0 + (piyn * z3) * Dz3);
-    const double I1nn = 2.0 * un * (pitn * Dut + pixn * Dux + piyn * Duy + pinn * Dun) -
-                        2.0 * z3 * (pitn * Dz0 + pinn * Dz3);
+    const R I1nn = R(2.0) * un * (pitn * Dut + pixn * Dux + piyn * Duy + pinn * Dun) -
+                        R(2.0) * z3 * (pitn * Dz0 + pinn * Dz3);
 
-    const double ps2 = 0.5 * psT;
-    const double I4tt = (pitt * sttT - pitx * stxT - pity * styT - t2 * pitn * stnT) - (1.0 - ut2 + z02) * ps2;
-    const double I4tx = 0.5 * ((pitt * stxT + pitx * sttT) - (pitx * sxxT + pixx * stxT) - (pity * sxyT + pixy * styT) -
+    const R ps2 = R(0.5) * psT;
+    const R I4tt = (pitt * sttT - pitx * stxT - pity * styT - t2 * pitn * stnT) - (R(1.0) - ut2 + z02) * ps2;
+    const R I4tx = R(0.5) * ((pitt * stxT + pitx * sttT) - (pitx * sxxT + pixx * stxT) - (pity * sxyT + pixy * styT) -
                                t2 * (pitn * sxnT + pixn * stnT)) + (ut * ux) * ps2;
-    const double I4ty = 0.5 * ((pitt * styT + pity * sttT) - (pitx * sxyT + pixy * stxT) - (pity * syyT + piyy * styT) -
+    const R I4ty = R(0.5) * ((pitt * styT + pity * sttT) - (pitx * sxyT + pixy * stxT) - (pity * syyT + piyy * styT) -
                                t2 * (pitn * synT + piyn * stnT)) + (ut * uy) * ps2;
-    const double I4tn = 0.5 * ((pitt * stnT + pitn * sttT) - (pitx * sxnT + pixn * stxT) - (pity * synT + piyn * styT) -
+    const R I4tn = R(0.5) * ((pitt * stnT + pitn * sttT) - (pitx * sxnT + pixn * stxT) - (pity * synT + piyn * styT) -
                                t2 * (pitn * snnT + pinn * stnT)) + (ut * un - z0z3) * ps2;
-    const double I4xx = (pitx * stxT - pixx * sxxT - pixy * sxyT - t2 * pixn * sxnT) + (1.0 + ux * ux) * ps2;
-    const double I4xy = 0.5 * ((pitx * styT + pity * stxT) - (pixx * sxyT + pixy * sxxT) - (pixy * syyT + piyy * sxyT) -
+    const R I4xx = (pitx * stxT - pixx * sxxT - pixy * sxyT - t2 * pixn * sxnT) + (R(1.0) + ux * ux) * ps2;
+    const R I4xy = R(0.5) * ((pitx * styT + pity * stxT) - (pixx * sxyT + pixy * sxxT) - (pixy * syyT + piyy * sxyT) -
                                t2 * (pixn * synT + piyn * sxnT)) + (ux * uy) * ps2;
-    const double I4xn = 0.5 * ((pitx * stnT + pitn * stxT) - (pixx * sxnT + pixn * sxxT) - (pixy * synT + piyn * sxyT) -
+    const R I4xn = R(0.5) * ((pitx * stnT + pitn * stxT) - (pixx * sxnT + pixn * sxxT) - (pixy * synT + piyn * sxyT) -
                                t2 * (pixn * snnT + pinn * sxnT)) + (ux * un) * ps2;
-    const double I4yy = (pity * styT - pixy * sxyT - piyy * syyT - t2 * piyn * synT) + (1.0 + uy * uy) * ps2;
-    const double I4yn = 0.5 * ((pity * stnT + pitn * styT) - (pixy * sxnT + pixn * sxyT) - (piyy * synT + piyn * syyT) -
+    const R I4yy = (pity * styT - pixy * sxyT - piyy * syyT - t2 * piyn * synT) + (R(1.0) + uy * uy) * ps2;
+    const R I4yn = R(0.5) * ((pity * stnT + pitn * styT) - (pixy * sxnT + pixn * sxyT) - (piyy * synT + piyn * syyT) -
                                t2 * (piyn * snnT + pinn * synT)) + (uy * un) * ps2;
-    const double I4nn = (pitn * stnT - pixn * sxnT - piyn * synT - t2 * pinn * snnT) + (it2 + un2 - z32) * ps2;
+    const R I4nn = (pitn * stnT - pixn * sxnT - piyn * synT - t2 * pinn * snnT) + (it2 + un2 - z32) * ps2;
 
     // I = I1 - delta_pipi I2 + tau_pipi I4 - lambda_pipi I5 with I2 = thetaT pi, I5 = zDzu pi
-    const double c25 = delta_pipi * thetaT + lambda_pipi * zDzu;
-    const double twoEta = 2.0 * etaBar_TC;
+    const R c25 = delta_pipi * thetaT + lambda_pipi * zDzu;
+    const R twoEta = R(2.0) * etaBar_TC;
 
-    const double dpitt = twoEta * sttT - pitt * taupiInv - (I1tt - c25 * pitt + tau_pipi * I4tt) - 2.0 * un * t * pitn;
-    const double dpitx = twoEta * stxT - pitx * taupiInv - (I1tx - c25 * pitx + tau_pipi * I4tx) - un * t * pixn;
-    const double dpity = twoEta * styT - pity * taupiInv - (I1ty - c25 * pity + tau_pipi * I4ty) - un * t * piyn;
-    const double dpitn = twoEta * stnT - pitn * taupiInv - (I1tn - c25 * pitn + tau_pipi * I4tn) - un * t * pinn -
+    const R dpitt = twoEta * sttT - pitt * taupiInv - (I1tt - c25 * pitt + tau_pipi * I4tt) - R(2.0) * un * t * pitn;
+    const R dpitx = twoEta * stxT - pitx * taupiInv - (I1tx - c25 * pitx + tau_pipi * I4tx) - un * t * pixn;
+    const R dpity = twoEta * styT - pity * taupiInv - (I1ty - c25 * pity + tau_pipi * I4ty) - un * t * piyn;
+    const R dpitn = twoEta * stnT - pitn * taupiInv - (I1tn - c25 * pitn + tau_pipi * I4tn) - un * t * pinn -
                          (ut * pitn + un * pitt) * it;
-    const double dpixx = twoEta * sxxT - pixx * taupiInv - (I1xx - c25 * pixx + tau_pipi * I4xx);
-    const double dpixy = twoEta * sxyT - pixy * taupiInv - (I1xy - c25 * pixy + tau_pipi * I4xy);
-    const double dpixn = twoEta * sxnT - pixn * taupiInv - (I1xn - c25 * pixn + tau_pipi * I4xn) -
+    const R dpixx = twoEta * sxxT - pixx * taupiInv - (I1xx - c25 * pixx + tau_pipi * I4xx);
+    const R dpixy = twoEta * sxyT - pixy * taupiInv - (I1xy - c25 * pixy + tau_pipi * I4xy);
+    const R dpixn = twoEta * sxnT - pixn * taupiInv - (I1xn - c25 * pixn + tau_pipi * I4xn) -
                          (ut * pixn + un * pitx) * it;
-    const double dpiyy = twoEta * syyT - piyy * taupiInv - (I1yy - c25 * piyy + tau_pipi * I4yy);
-    const double dpiyn = twoEta * synT - piyn * taupiInv - (I1yn - c25 * piyn + tau_pipi * I4yn) -
+    const R dpiyy = twoEta * syyT - piyy * taupiInv - (I1yy - c25 * piyy + tau_pipi * I4yy);
+    const R dpiyn = twoEta * synT - piyn * taupiInv - (I1yn - c25 * piyn + tau_pipi * I4yn) -
                          (ut * piyn + un * pity) * it;
-    const double dpinn = twoEta * snnT - pinn * taupiInv - (I1nn - c25 * pinn + tau_pipi * I4nn) -
-                         2.0 * (ut * pinn + un * pitn) * it;
+    const R dpinn = twoEta * snnT - pinn * taupiInv - (I1nn - c25 * pinn + tau_pipi * I4nn) -
+                         R(2.0) * (ut * pinn + un * pitn) * it;
 
     S[PITT] = dpitt * iut + pitt * dkvk;
     S[PITX] = dpitx * iut + pitx * dkvk;
@@ -252,13 +253,14 @@ __device__ __forceinline__ void source_terms(const Eos &E, const SrcIn &in, cons
 
 // Axis sources from the pi gradients (SourceTerms.cpp:46-176):
 // d = (pi(+1) - pi(-1)) * (1/d)/2 of the listed components, v = u^i / u^tau.
+template <typename R>
 struct AxisPi {
-    double dtt, dtx, dty, dtn;  // d_i pi^{tau mu}
-    double dix, diy, din;       // d_i pi^{i mu} for mu = x, y, eta (direction-dependent set)
+    R dtt, dtx, dty, dtn;  // d_i pi^{tau mu}
+    R dix, diy, din;       // d_i pi^{i mu} for mu = x, y, eta (direction-dependent set)
 };
 
-template <int DIR>
-__device__ __forceinline__ void axis_source(const AxisPi &d, double v, double (&S)[4]) {
+template <int DIR, typename R>
+__device__ __forceinline__ void axis_source(const AxisPi<R> &d, R v, R (&S)[4]) {
     // DIR 1: pi^{x mu} = (pitx, pixx, pixy, pixn); DIR 2: (pity, pixy, piyy, piyn); DIR 3: (pitn, pixn, piyn, pinn)
     S[0] = d.dtt * v - (DIR == 1 ? d.dtx : DIR == 2 ? d.dty : d.dtn);
     S[1] = d.dtx * v - d.dix;

@@ -26,6 +26,14 @@
 
 namespace vah {
 
+// workgroups per CU the register allocation is sized for (occupancy target)
+#ifndef VAH_BLOCKS_2D
+#define VAH_BLOCKS_2D 2
+#endif
+#ifndef VAH_BLOCKS_3D
+#define VAH_BLOCKS_3D 2
+#endif
+
 constexpr int TX = 16;
 constexpr int TY = 15;
 constexpr int NTHREADS = 256;
@@ -52,6 +60,7 @@ struct StageParams {
     double hx, hy, hz;              // (1/d)/2, the reference's centred-difference factor
     Eos eos;
     unsigned int *counters;  // [0] NaN cells, [1] recovery fallbacks
+    unsigned long long *emax;  // running max of the new e (ordered bits), or null
 };
 
 __device__ __forceinline__ int remap_block(int b, int nb) {
@@ -60,30 +69,122 @@ __device__ __forceinline__ int remap_block(int b, int nb) {
     return (b & 7) * per + (b >> 3);
 }
 
-// Reconstruct the two sides of the face between cells c-st and c and
-// solve the KT flux (SemiDiscreteKurganovTadmorScheme.cpp:13-57).
+// ---------------------------------------------------------------------------
+// KT flux through one cell face (SemiDiscreteKurganovTadmorScheme.cpp:13-57)
+// ---------------------------------------------------------------------------
+
+// Newton inputs of one reconstructed face state (EnergyMomentumTensor.cpp:116-143
+// with W = Pi = 0): M^mu = T^{tau mu} - pi^{tau mu}, F = M^3/(M^0 + P_L).
+struct FaceState {
+    double M0, M1, M2, M3, PL, F, Ft, x, MBT;
+};
+
+__device__ __forceinline__ FaceState face_state(double t, double ttt, double ttx, double tty, double ttn, double pl,
+                                                double pitt, double pitx, double pity, double pitn) {
+    FaceState f;
+    f.M0 = ttt - pitt;
+    f.M1 = ttx - pitx;
+    f.M2 = tty - pity;
+    f.M3 = ttn - pitn;
+    f.PL = pl;
+    double M0PL = f.M0 + pl;
+    if (M0PL == 0.0) M0PL = 1.e-16;
+    f.F = fdiv(f.M3, M0PL);
+    f.Ft = t * f.F;
+    f.x = __builtin_sqrt(fabs(1.0 - f.Ft * f.Ft));
+    f.MBT = __builtin_sqrt(f.M1 * f.M1 + f.M2 * f.M2);
+    if (f.MBT == 0.0) f.MBT = 1.e-16;
+    return f;
+}
+
+// One Newton update of u_T for state S from u0 (EnergyMomentumTensor.cpp:45-70).
+// Returns 1 converged, -1 failed, 0 continue; *u1 receives the update.
+__device__ __forceinline__ int newton_update(double t, const FaceState &S, double u0, double *u1) {
+    const double Ft2 = S.Ft * S.Ft;
+    const double xM = S.x * S.MBT;
+    const double bL = S.x * S.x * S.M0 - Ft2 * S.PL;
+    const double b = S.x * S.x + Ft2;
+    const double c0 = S.M0 - t * S.Ft * S.M3;
+    double rs = rsqrt(1.0 + u0 * u0);
+    double U = u0 * rs;
+    double e = c0 - U * xM;
+    if (e < 0.0) return -1;
+    double iP = rs * rs * rs;
+    double g = bL + b * 0.5 * (e - S.PL);
+    double f = U * g - xM;
+    double fp = iP * g - 0.5 * U * b * xM * iP;
+    if (fabs(fp) == 0.0) fp = 1.e-16;
+    double u = u0 - fdiv(f, fp);
+    *u1 = u;
+    if (__builtin_isnan(u) || __builtin_isinf(u) || u < 0.0 || u > 9.0072e+15) return -1;
+    return fabs(u - u0) <= 1.e-7 * fabs(u) ? 1 : 0;
+}
+
+// Both sides of a face solved in one loop (independent iterations, each side
+// stops at its own convergence or failure, at most 10^4 iterations each).
+// Outputs the signed speed v = u^DIR / u^tau of each side; a failed recovery
+// gives the reference's fallback u = (1,0,0,0), i.e. v = 0.
 template <int DIR>
-__device__ __forceinline__ void face_flux(const Eos &E, double t, double uT_guess, const double *__restrict__ q,
-                                          long fs, long c, long st, double (&H)[NQ], bool &nan) {
-    double qL[NQ], qR[NQ];
-#pragma unroll
-    for (int n = 0; n < NQ; ++n) {
-        const double *a = q + n * fs;
-        double q0 = a[c - 2 * st], q1 = a[c - st], q2 = a[c], q3 = a[c + st];
-        qL[n] = q1 + 0.5 * slope(q0, q1, q2);
-        qR[n] = q2 - 0.5 * slope(q1, q2, q3);
+__device__ __forceinline__ void face_speeds(double t, double guess, const FaceState &A, const FaceState &B, double &vA,
+                                            double &vB) {
+    double a0 = guess, b0 = guess, ua = 0.0, ub = 0.0;
+    int sa = 0, sb = 0;
+    for (int it = 0; it < 10000 && (sa == 0 || sb == 0); ++it) {
+        if (sa == 0) {
+            sa = newton_update(t, A, a0, &ua);
+            a0 = ua;
+        }
+        if (sb == 0) {
+            sb = newton_update(t, B, b0, &ub);
+            b0 = ub;
+        }
     }
-    Prim R = recover(E, t, qR, 0.0, uT_guess);
-    Prim L = recover(E, t, qL, 0.0, uT_guess);
-    nan = nan || prim_nan(R) || prim_nan(L);
-    double uiR = DIR == 1 ? R.ux : DIR == 2 ? R.uy : R.un;
-    double uiL = DIR == 1 ? L.ux : DIR == 2 ? L.uy : L.un;
-    double vR = uiR * rcp(R.ut), vL = uiL * rcp(L.ut);
-    double a = fmax(fabs(vL), fabs(vR));
-    // (F(qR) + F(qL) - a (qR - qL)) / 2 with F(q) = v q
-    double cR = 0.5 * (vR - a), cL = 0.5 * (vL + a);
+    auto speed = [](const FaceState &S, double u, int st) {
+        if (st != 1) return 0.0;
+        if (DIR == 3) return S.F;  // u^eta / u^tau = F C/x / (C/x)
+        double iC = rsqrt(1.0 + u * u);
+        double num = DIR == 1 ? S.M1 : S.M2;
+        return u * num * S.x * iC * rcp(S.MBT);  // (u M/MBT) / (C/x)
+    };
+    vA = speed(A, ua, sa);
+    vB = speed(B, ub, sb);
+}
+
+// Flux through the face between cells c-st and c.  The left/right states are
+// the reference's half-cell extrapolations (HalfSiteExtrapolation.cpp:12-26);
+// emit(n, H_n) receives the 15 components in order.  The nine components the
+// Newton solve needs stay in registers, the other six are reconstructed after
+// it, which keeps the live state small.
+template <int DIR, typename R, typename Emit>
+__device__ __forceinline__ void face_flux(double t, double uT_guess, const double *__restrict__ q, long fs, long c,
+                                          long st, Emit emit) {
+    constexpr int NN = 9;  // TTT..PITN
+    R qL[NN], qR[NN];
 #pragma unroll
-    for (int n = 0; n < NQ; ++n) H[n] = cR * qR[n] + cL * qL[n];
+    for (int n = 0; n < NN; ++n) {
+        const double *a = q + n * fs;
+        R q0 = (R)a[c - 2 * st], q1 = (R)a[c - st], q2 = (R)a[c], q3 = (R)a[c + st];
+        qL[n] = q1 + R(0.5) * slope(q0, q1, q2);
+        qR[n] = q2 - R(0.5) * slope(q1, q2, q3);
+    }
+    // the Newton recovery of both sides runs in double whatever R is
+    FaceState FR = face_state(t, qR[TTT], qR[TTX], qR[TTY], qR[TTN], qR[PL], qR[PITT], qR[PITX], qR[PITY], qR[PITN]);
+    FaceState FL = face_state(t, qL[TTT], qL[TTX], qL[TTY], qL[TTN], qL[PL], qL[PITT], qL[PITX], qL[PITY], qL[PITN]);
+    double vR, vL;
+    face_speeds<DIR>(t, uT_guess, FR, FL, vR, vL);
+    const double a = fmax(fabs(vL), fabs(vR));
+    // (F(qR) + F(qL) - a (qR - qL)) / 2 with F(q) = v q
+    const R cR = (R)(0.5 * (vR - a)), cL = (R)(0.5 * (vL + a));
+#pragma unroll
+    for (int n = 0; n < NN; ++n) emit(n, cR * qR[n] + cL * qL[n]);
+#pragma unroll
+    for (int n = NN; n < NQ; ++n) {
+        const double *a = q + n * fs;
+        R q0 = (R)a[c - 2 * st], q1 = (R)a[c - st], q2 = (R)a[c], q3 = (R)a[c + st];
+        R l = q1 + R(0.5) * slope(q0, q1, q2);
+        R r = q2 - R(0.5) * slope(q1, q2, q3);
+        emit(n, cR * r + cL * l);
+    }
 }
 
 __device__ __forceinline__ double uT_of(const double *__restrict__ u, long fs, long s) {
@@ -92,10 +193,10 @@ __device__ __forceinline__ double uT_of(const double *__restrict__ u, long fs, l
 }
 
 // d_i of the pi components entering the axis source of direction DIR
-template <int DIR>
-__device__ __forceinline__ AxisPi axis_pi(const double *__restrict__ q, long fs, long s, long st, double h) {
-    auto d = [&](int n) { return (q[n * fs + s + st] - q[n * fs + s - st]) * h; };
-    AxisPi r;
+template <int DIR, typename R>
+__device__ __forceinline__ AxisPi<R> axis_pi(const double *__restrict__ q, long fs, long s, long st, double h) {
+    auto d = [&](int n) { return (R)((q[n * fs + s + st] - q[n * fs + s - st]) * h); };
+    AxisPi<R> r;
     r.dtt = d(PITT);
     r.dtx = d(PITX);
     r.dty = d(PITY);
@@ -116,19 +217,14 @@ __device__ __forceinline__ AxisPi axis_pi(const double *__restrict__ q, long fs,
     return r;
 }
 
-// acc += dt * [ (H- - H+)/d + S_axis ]   (the reference adds the axis source
-// to the four T^{tau mu} components only)
-template <int DIR>
-__device__ __forceinline__ void add_direction(double (&acc)[NQ], const double (&Hm)[NQ], const double (&Hp)[NQ],
-                                              const AxisPi &ap, double v, double inv_d, double dt) {
-    double Sa[4];
-    axis_source<DIR>(ap, v, Sa);
-#pragma unroll
-    for (int n = 0; n < NQ; ++n) {
-        double r = (Hm[n] - Hp[n]) * inv_d;
-        if (n < 4) r += Sa[n];
-        acc[n] += r * dt;
-    }
+// one component of the reference's per-direction update:
+// acc += dt * [ (H- - H+)/d  (+ S_axis for the four T^{tau mu} components) ]
+// (the difference in R, the accumulation in double)
+template <typename R>
+__device__ __forceinline__ void add_component(double &acc, int n, R Hm, R Hp, const R (&Sa)[4], R inv_d, double dt) {
+    R r = (Hm - Hp) * inv_d;
+    if (n < 4) r += Sa[n];
+    acc += (double)r * dt;
 }
 
 template <bool D3>
@@ -181,7 +277,7 @@ __device__ __forceinline__ void load_src_in(const StageParams &P, long s, SrcIn 
 
 // Stage-2 convex combination, recovery, regulation, store with ghost copies.
 template <bool D3, bool STAGE2>
-__device__ __forceinline__ void finish_cell(const StageParams &P, long s, int ii, int jj, int kk, double (&acc)[NQ],
+__device__ __forceinline__ double finish_cell(const StageParams &P, long s, int ii, int jj, int kk, double (&acc)[NQ],
                                             bool &nan, bool &fallback) {
     const long fs = P.fs;
     if (STAGE2) {
@@ -220,23 +316,33 @@ __device__ __forceinline__ void finish_cell(const StageParams &P, long s, int ii
         if (kk == 0) { store(s - P.sz); store(s - 2 * P.sz); }
         if (kk == P.nz - 1) { store(s + P.sz); store(s + 2 * P.sz); }
     }
+    return e;
 }
 
-__device__ __forceinline__ void count_flags(const StageParams &P, bool nan, bool fallback) {
+// NaN checks of the reference (EnergyMomentumTensor.cpp:178-194 and
+// SourceTerms.cpp:658-719 exit there) become a counter the host reads.
+// The wave's largest new e goes to P.emax (as ordered bits: e > 0) when set,
+// which the batch uses for its freezeout test.
+__device__ __forceinline__ void count_flags(const StageParams &P, bool nan, bool fallback, double e) {
     unsigned long long bn = __ballot(nan), bf = __ballot(fallback);
     if ((threadIdx.x & 63) == 0) {
         if (bn) atomicAdd(&P.counters[0], (unsigned)__popcll(bn));
         if (bf) atomicAdd(&P.counters[1], (unsigned)__popcll(bf));
     }
+    if (P.emax) {
+        double m = e > 0.0 ? e : 0.0;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+        if ((threadIdx.x & 63) == 0 && m > 0.0) atomicMax(P.emax, (unsigned long long)__double_as_longlong(m));
+    }
 }
 
-template <bool D3, bool STAGE2, bool SHARED>
-__global__ __launch_bounds__(NTHREADS, 2) void stage_kernel(StageParams P) {
-    __shared__ double Hs[SHARED ? NQ : 1][SHARED ? NTHREADS : 1];
-
-    const int b = (int)blockIdx.x;
-    const int tile = (gridDim.x & 7) == 0 ? remap_block(b, (int)gridDim.x) : b;
-    if (tile >= P.nblocks) return;  // padding workgroups: no barrier reached
+// One tile of one stage.  Hs holds the x / y face fluxes of the tile
+// (SHARED) or this lane's forward face (reference split); Hz is the eta face
+// carried from the plane below, one slot per lane that only its own lane
+// touches.  R is the arithmetic type of the stencil, fluxes and sources.
+template <bool D3, bool STAGE2, bool SHARED, typename R>
+__device__ __forceinline__ void stage_tile(const StageParams &P, int tile, R (*Hs)[NTHREADS], R (*Hz)[NTHREADS]) {
     const int tpl = P.tiles_x * P.tiles_y;
     const int zc = tile / tpl;
     const int txy = tile - zc * tpl;
@@ -248,14 +354,13 @@ __global__ __launch_bounds__(NTHREADS, 2) void stage_kernel(StageParams P) {
 
     const int tid = (int)threadIdx.x;
     const long fs = P.fs, sy = P.sy, sz = P.sz;
-    const Eos &E = P.eos;
+    const double t = P.t_src, dt = P.dt;
+    const R inv_dx = (R)P.inv_dx, inv_dy = (R)P.inv_dy, inv_dz = (R)P.inv_dz;
 
     // this lane's cell
     const int ci = tid % TX, cj = tid / TX;
     const int ii = i0 + ci, jj = j0 + cj;
     const bool cell_ok = cj < TY && ii < P.nx && jj < P.ny;
-
-    double Hzm[NQ];  // eta face below the current plane (D3 && SHARED)
 
     for (int kk = k0; kk < k1; ++kk) {
         const long plane = (long)(kk + 2) * sz;
@@ -263,107 +368,118 @@ __global__ __launch_bounds__(NTHREADS, 2) void stage_kernel(StageParams P) {
         bool nan = false, fallback = false;
         double acc[NQ];
 
+        // ---- sources: acc = q + dt S (eulerStepKernelSource)
         if (cell_ok) {
             double Q[NQ];
 #pragma unroll
             for (int n = 0; n < NQ; ++n) Q[n] = P.qin[n * fs + s];
             SrcIn in;
             load_src_in<D3>(P, s, in);
-            double S[NQ];
-            source_terms(E, in, Q, S, nan);
+            R S[NQ];
+            source_terms<R>(P.eos, in, Q, S, nan);
 #pragma unroll
-            for (int n = 0; n < NQ; ++n) acc[n] = Q[n] + P.dt * S[n];
+            for (int n = 0; n < NQ; ++n) acc[n] = Q[n] + dt * (double)S[n];
         }
+        const double iut = cell_ok ? rcp(P.ucur[s]) : 0.0;
 
         if (SHARED) {
-            // ---- x faces: 17 x 15 faces, face f sits between cells (fi-1, fj) and (fi, fj)
+            // ---- x faces: 17 x 15, face f between cells (fi-1, fj) and (fi, fj)
             {
                 const int fi = tid % (TX + 1), fj = tid / (TX + 1);
                 const int gi = i0 + fi, gj = j0 + fj;  // interior index of the right cell
                 if (fj < TY && gi <= P.nx && gj < P.ny) {
                     const long c = plane + (long)(gj + 2) * sy + (gi + 2);
-                    double H[NQ];
-                    face_flux<1>(E, P.t_src, uT_of(P.ucur, fs, c - 1), P.qin, fs, c, 1, H, nan);
-#pragma unroll
-                    for (int n = 0; n < NQ; ++n) Hs[n][tid] = H[n];
+                    face_flux<1, R>(t, uT_of(P.ucur, fs, c - 1), P.qin, fs, c, 1,
+                                    [&](int n, R h) { Hs[n][tid] = h; });
                 }
             }
             __syncthreads();
             if (cell_ok) {
-                double Hm[NQ], Hp[NQ];
                 const int fm = cj * (TX + 1) + ci;
+                R Sa[4];
+                axis_source<1>(axis_pi<1, R>(P.qin, fs, s, 1, P.hx), (R)(P.ucur[fs + s] * iut), Sa);
 #pragma unroll
-                for (int n = 0; n < NQ; ++n) {
-                    Hm[n] = Hs[n][fm];
-                    Hp[n] = Hs[n][fm + 1];
-                }
-                AxisPi ap = axis_pi<1>(P.qin, fs, s, 1, P.hx);
-                add_direction<1>(acc, Hm, Hp, ap, P.ucur[fs + s] * rcp(P.ucur[s]), P.inv_dx, P.dt);
+                for (int n = 0; n < NQ; ++n) add_component(acc[n], n, Hs[n][fm], Hs[n][fm + 1], Sa, inv_dx, dt);
             }
             __syncthreads();
-            // ---- y faces: 16 x 16 faces, face f sits between cells (fi, fj-1) and (fi, fj)
+            // ---- y faces: 16 x 16, face f between cells (fi, fj-1) and (fi, fj)
             {
                 const int fi = tid % TX, fj = tid / TX;
                 const int gi = i0 + fi, gj = j0 + fj;
                 if (fj <= TY && gi < P.nx && gj <= P.ny) {
                     const long c = plane + (long)(gj + 2) * sy + (gi + 2);
-                    double H[NQ];
-                    face_flux<2>(E, P.t_src, uT_of(P.ucur, fs, c - sy), P.qin, fs, c, sy, H, nan);
-#pragma unroll
-                    for (int n = 0; n < NQ; ++n) Hs[n][tid] = H[n];
+                    face_flux<2, R>(t, uT_of(P.ucur, fs, c - sy), P.qin, fs, c, sy,
+                                    [&](int n, R h) { Hs[n][tid] = h; });
                 }
             }
             __syncthreads();
             if (cell_ok) {
-                double Hm[NQ], Hp[NQ];
                 const int fm = cj * TX + ci;
+                R Sa[4];
+                axis_source<2>(axis_pi<2, R>(P.qin, fs, s, sy, P.hy), (R)(P.ucur[2 * fs + s] * iut), Sa);
 #pragma unroll
-                for (int n = 0; n < NQ; ++n) {
-                    Hm[n] = Hs[n][fm];
-                    Hp[n] = Hs[n][fm + TX];
-                }
-                AxisPi ap = axis_pi<2>(P.qin, fs, s, sy, P.hy);
-                add_direction<2>(acc, Hm, Hp, ap, P.ucur[2 * fs + s] * rcp(P.ucur[s]), P.inv_dy, P.dt);
+                for (int n = 0; n < NQ; ++n) add_component(acc[n], n, Hs[n][fm], Hs[n][fm + TX], Sa, inv_dy, dt);
             }
             __syncthreads();
+            // ---- eta faces: solved by the cell's own lane, the upper one carried to the next plane
             if (D3 && cell_ok) {
-                if (kk == k0) face_flux<3>(E, P.t_src, uT_of(P.ucur, fs, s - sz), P.qin, fs, s, sz, Hzm, nan);
-                double Hzp[NQ];
-                face_flux<3>(E, P.t_src, uT_of(P.ucur, fs, s), P.qin, fs, s + sz, sz, Hzp, nan);
-                AxisPi ap = axis_pi<3>(P.qin, fs, s, sz, P.hz);
-                add_direction<3>(acc, Hzm, Hzp, ap, P.ucur[3 * fs + s] * rcp(P.ucur[s]), P.inv_dz, P.dt);
-#pragma unroll
-                for (int n = 0; n < NQ; ++n) Hzm[n] = Hzp[n];
+                if (kk == k0)
+                    face_flux<3, R>(t, uT_of(P.ucur, fs, s - sz), P.qin, fs, s, sz,
+                                    [&](int n, R h) { Hz[n][tid] = h; });
+                R Sa[4];
+                axis_source<3>(axis_pi<3, R>(P.qin, fs, s, sz, P.hz), (R)(P.ucur[3 * fs + s] * iut), Sa);
+                face_flux<3, R>(t, uT_of(P.ucur, fs, s), P.qin, fs, s + sz, sz, [&](int n, R h) {
+                    add_component(acc[n], n, Hz[n][tid], h, Sa, inv_dz, dt);
+                    Hz[n][tid] = h;
+                });
             }
         } else if (cell_ok) {
-            // reference work split: both faces per direction, Newton guess from this cell
+            // reference work split: both faces of each direction, Newton guess from this cell
             const double uTs = uT_of(P.ucur, fs, s);
             {
-                double Hm[NQ], Hp[NQ];
-                face_flux<1>(E, P.t_src, uTs, P.qin, fs, s + 1, 1, Hp, nan);
-                face_flux<1>(E, P.t_src, uTs, P.qin, fs, s, 1, Hm, nan);
-                AxisPi ap = axis_pi<1>(P.qin, fs, s, 1, P.hx);
-                add_direction<1>(acc, Hm, Hp, ap, P.ucur[fs + s] * rcp(P.ucur[s]), P.inv_dx, P.dt);
+                R Sa[4];
+                axis_source<1>(axis_pi<1, R>(P.qin, fs, s, 1, P.hx), (R)(P.ucur[fs + s] * iut), Sa);
+                face_flux<1, R>(t, uTs, P.qin, fs, s + 1, 1, [&](int n, R h) { Hs[n][tid] = h; });
+                face_flux<1, R>(t, uTs, P.qin, fs, s, 1,
+                                [&](int n, R h) { add_component(acc[n], n, h, Hs[n][tid], Sa, inv_dx, dt); });
             }
             {
-                double Hm[NQ], Hp[NQ];
-                face_flux<2>(E, P.t_src, uTs, P.qin, fs, s + sy, sy, Hp, nan);
-                face_flux<2>(E, P.t_src, uTs, P.qin, fs, s, sy, Hm, nan);
-                AxisPi ap = axis_pi<2>(P.qin, fs, s, sy, P.hy);
-                add_direction<2>(acc, Hm, Hp, ap, P.ucur[2 * fs + s] * rcp(P.ucur[s]), P.inv_dy, P.dt);
+                R Sa[4];
+                axis_source<2>(axis_pi<2, R>(P.qin, fs, s, sy, P.hy), (R)(P.ucur[2 * fs + s] * iut), Sa);
+                face_flux<2, R>(t, uTs, P.qin, fs, s + sy, sy, [&](int n, R h) { Hs[n][tid] = h; });
+                face_flux<2, R>(t, uTs, P.qin, fs, s, sy,
+                                [&](int n, R h) { add_component(acc[n], n, h, Hs[n][tid], Sa, inv_dy, dt); });
             }
             if (D3) {
-                double Hm[NQ], Hp[NQ];
-                face_flux<3>(E, P.t_src, uTs, P.qin, fs, s + sz, sz, Hp, nan);
-                face_flux<3>(E, P.t_src, uTs, P.qin, fs, s, sz, Hm, nan);
-                AxisPi ap = axis_pi<3>(P.qin, fs, s, sz, P.hz);
-                add_direction<3>(acc, Hm, Hp, ap, P.ucur[3 * fs + s] * rcp(P.ucur[s]), P.inv_dz, P.dt);
+                R Sa[4];
+                axis_source<3>(axis_pi<3, R>(P.qin, fs, s, sz, P.hz), (R)(P.ucur[3 * fs + s] * iut), Sa);
+                face_flux<3, R>(t, uTs, P.qin, fs, s + sz, sz, [&](int n, R h) { Hs[n][tid] = h; });
+                face_flux<3, R>(t, uTs, P.qin, fs, s, sz,
+                                [&](int n, R h) { add_component(acc[n], n, h, Hs[n][tid], Sa, inv_dz, dt); });
             }
         }
 
-        if (cell_ok) finish_cell<D3, STAGE2>(P, s, ii, jj, kk, acc, nan, fallback);
-        count_flags(P, nan, fallback);
+        double e_new = 0.0;
+        if (cell_ok) e_new = finish_cell<D3, STAGE2>(P, s, ii, jj, kk, acc, nan, fallback);
+        count_flags(P, nan, fallback, e_new);
     }
+}
+
+// The LDS of a stage workgroup, sized by the template arguments.
+template <bool D3, bool SHARED, typename R>
+struct StageLds {
+    R Hs[NQ][NTHREADS];
+    R Hz[D3 && SHARED ? NQ : 1][NTHREADS];
+};
+
+// One event: workgroup -> tile (XCD-aware) -> stage_tile.
+template <bool D3, bool STAGE2, bool SHARED, typename R>
+__global__ __launch_bounds__(NTHREADS, D3 ? VAH_BLOCKS_3D : VAH_BLOCKS_2D) void stage_kernel(StageParams P) {
+    __shared__ StageLds<D3, SHARED, R> lds;
+    const int b = (int)blockIdx.x;
+    const int tile = (gridDim.x & 7) == 0 ? remap_block(b, (int)gridDim.x) : b;
+    if (tile >= P.nblocks) return;  // padding workgroups: no barrier reached
+    stage_tile<D3, STAGE2, SHARED, R>(P, tile, lds.Hs, lds.Hz);
 }
 
 // Zero-gradient ghost copies of an initial state: each boundary interior cell

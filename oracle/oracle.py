@@ -141,32 +141,37 @@ class Oracle:
             pass
 
     def set_threads(self, n):
-        lib().orc_set_threads(self.h, n)
+        lib().orc_set_threads(self._handle(), n)
 
     def set_primitives(self, e, ut, ux, uy, un, pl=None, pi=None):
         c = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float64).ravel()
         e, ut, ux, uy, un, pl, pi = map(c, (e, ut, ux, uy, un, pl, pi))
-        lib().orc_set_primitives(self.h, _ptr(e), _ptr(pl), _ptr(ut), _ptr(ux), _ptr(uy), _ptr(un), _ptr(pi))
+        lib().orc_set_primitives(self._handle(), _ptr(e), _ptr(pl), _ptr(ut), _ptr(ux), _ptr(uy), _ptr(un), _ptr(pi))
 
     def set_state(self, nstep, q, e, p, u, up):
         c = lambda a: np.ascontiguousarray(a, dtype=np.float64).ravel()
         q, e, p, u, up = map(c, (q, e, p, u, up))
-        lib().orc_set_state(self.h, nstep, _ptr(q), _ptr(e), _ptr(p), _ptr(u), _ptr(up))
+        lib().orc_set_state(self._handle(), nstep, _ptr(q), _ptr(e), _ptr(p), _ptr(u), _ptr(up))
+
+    def _handle(self):
+        if not self.h:
+            raise RuntimeError("oracle lattice already closed")
+        return self.h
 
     def step(self, n=1):
-        lib().orc_step(self.h, n)
+        lib().orc_step(self._handle(), n)
 
     @property
     def time(self):
-        return lib().orc_time(self.h)
+        return lib().orc_time(self._handle())
 
     @property
     def newton_fail(self):
-        return lib().orc_newton_fail(self.h)
+        return lib().orc_newton_fail(self._handle())
 
     @property
     def nan_hits(self):
-        return lib().orc_nan_hits(self.h)
+        return lib().orc_nan_hits(self._handle())
 
     def state(self):
         n = self.n
@@ -175,5 +180,5 @@ class Oracle:
         p = np.empty(n)
         u = np.empty(4 * n)
         up = np.empty(4 * n)
-        lib().orc_get_state(self.h, _ptr(q), _ptr(e), _ptr(p), _ptr(u), _ptr(up))
+        lib().orc_get_state(self._handle(), _ptr(q), _ptr(e), _ptr(p), _ptr(u), _ptr(up))
         return {"q": q.reshape(NQ, n), "e": e, "p": p, "u": u.reshape(4, n), "up": up.reshape(4, n)}

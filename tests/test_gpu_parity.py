@@ -21,10 +21,10 @@ def run_pair(oracle_mod, nx, ny, nz, steps, e, u=None, pl=None, pi=None, dx=0.1,
     g.synchronize()
     o.step(steps)
     assert abs(g.time - o.time) == 0.0
-    a, b = g.state(), o.state()
+    a, b, tau = g.state(), o.state(), o.time
     g.close()
     o.close()
-    return a, b, o.time
+    return a, b, tau
 
 
 @pytest.mark.parametrize("flux_mode", [0, 1], ids=["shared", "per_cell"])
@@ -80,3 +80,59 @@ def test_ghost_cells_match_neighbours(oracle_mod):
     E = a["e"].reshape(ny, nx)
     assert np.all(E == E[0])  # no y structure may appear
     assert_parity(a, b, tau, label="ghosts")
+
+
+def test_lqcd_table_eos_3d(oracle_mod):
+    """The tabulated crossover EOS (10^4 nodes, linear in ln e) on a 3+1D lattice."""
+    V = vah()
+    table = V.eos_lqcd_table(10000, 1e-8, 1e4)
+    e = gaussian_ic(20, 18, 10, dz=0.3, seed=4, peak=60.0)
+    a, b, tau = run_pair(oracle_mod, 20, 18, 10, 20, e, pl=e / 3, dz=0.3, dt=0.02, etabar=0.16, eos_table=table)
+    assert_parity(a, b, tau, label="table EOS")
+
+
+def precision_errors(a, b):
+    """Per-field max and RMS relative errors of the split-precision run a
+    against the FP64 reference b (e and u^mu relative to u^tau, pi to e)."""
+    out = {}
+    e = b["e"]
+    out["e"] = np.abs(a["e"] - e) / e
+    for d, name in enumerate(("ut", "ux", "uy", "un")):
+        out[name] = np.abs(a["u"][d] - b["u"][d]) / b["u"][0]
+    out["pl"] = np.abs(a["q"][4] - b["q"][4]) / b["q"][4]
+    out["pi"] = np.max(np.abs(a["q"][5:] - b["q"][5:]), axis=0) / e
+    return {k: (float(v.max()), float(np.sqrt(np.mean(v * v)))) for k, v in out.items()}
+
+
+def test_fp32_split_precision_form(oracle_mod):
+    """C1's event with seeded +-5% lognormal noise on e, run to tau = 5 fm/c
+    (450 steps) with stencil, fluxes and sources in float: e and u^mu within
+    1e-4 of the FP64 oracle, total energy within 1e-6."""
+    V = vah()
+    nx = ny = 81
+    n = nx * ny
+    g0 = V.glauber()
+    e = V.ic_optical_glauber(nx, ny, 1, 0.1, 0.1, g0)
+    rng = np.random.default_rng(2024)
+    e = e * np.exp(0.05 * rng.standard_normal(n))
+    ut, ux, uy, un = rest_frame(n)
+    g = V.Vah(nx, ny, 1, 0.1, 0.1, 0.1, 0.01, 0.2, 0.5, precision=V.FP32_SPLIT)
+    o = oracle_mod.Oracle(nx, ny, 1, 0.1, 0.1, 0.1, 0.01, 0.2, 0.5)
+    g.set_primitives(e, ut, ux, uy, un, pl=e / 3)
+    o.set_primitives(e, ut, ux, uy, un, pl=e / 3)
+    g.step(450)
+    g.synchronize()
+    o.step(450)
+    assert o.time == pytest.approx(5.0)
+    a, b = g.state(), o.state()
+    errs = precision_errors(a, b)
+    print("split precision vs FP64 oracle (max, rms):", errs)
+    E_a = np.sum(a["q"][0]) * o.time
+    E_b = np.sum(b["q"][0]) * o.time
+    print("total energy drift", abs(E_a / E_b - 1))
+    assert errs["e"][0] < 1e-4
+    for k in ("ut", "ux", "uy", "un"):
+        assert errs[k][0] < 1e-4, k
+    assert abs(E_a / E_b - 1) < 1e-6
+    g.close()
+    o.close()

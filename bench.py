@@ -75,6 +75,229 @@ def cpu_baseline(w, e, eos_table, seconds=20.0):
             "sample": f"{steps} full-lattice steps of the same workload on the CPU oracle ({t_used:.1f} s)"}
 
 
+# ---------------------------------------------------------------------------
+# Batched (C4) and serving (C5) forms
+# ---------------------------------------------------------------------------
+HBARC = 0.197326938
+T_FREEZE_GEV = 0.150
+
+
+def mc_event(V, L, nz, dx, dz, b, seed):
+    """MC-Glauber Pb+Pb energy density (peak 60 GeV/fm^3) on an L x L x nz lattice;
+    an empty draw (no wounded nucleon at large b) is redrawn with the next seed."""
+    while True:
+        g = V.glauber(A=208, R=6.62, a=0.546, sigma_nn_mb=70.0, b=b, seed=seed, e_peak=60.0 / HBARC,
+                      nucleon_width=0.5, eta_flat=1.5, eta_sigma=1.4, e_floor=1e-3)
+        e, nw = V.ic_mc_glauber(L, L, nz, dx, dz, g)
+        if nw > 0:
+            return e
+        seed += 7919
+
+
+def parallel_map(fn, items, workers=16):
+    from concurrent.futures import ThreadPoolExecutor  # ctypes calls release the GIL
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        return list(ex.map(fn, items))
+
+
+def c4_events(V, n, seed):
+    """n boost-invariant events: side ~ U{101..261}, dx ~ U[0.08, 0.12],
+    dtau = 0.2 dx, b ~ U[0, 12] fm, tau0 = 0.5, eta/s = 0.16."""
+    rng = np.random.default_rng(seed)
+    evs = []
+    for i in range(n):
+        L = int(rng.integers(101, 262))
+        dx = float(rng.uniform(0.08, 0.12))
+        evs.append(dict(nx=L, ny=L, nz=1, dx=dx, dz=0.1, dt=0.2 * dx, t0=0.5, etabar=0.16,
+                        b=float(rng.uniform(0.0, 12.0)), seed=seed * 100003 + i))
+    es = parallel_map(lambda ev: mc_event(V, ev["nx"], 1, ev["dx"], ev["dz"], ev["b"], ev["seed"]), evs)
+    for ev, e in zip(evs, es):
+        ev["e"] = e
+    return evs
+
+
+def admit_event(B, ev, max_steps=0, tag=0):
+    return B.admit(ev["nx"], ev["ny"], ev["nz"], ev["dx"], ev["dx"], ev["dz"], ev["dt"], ev["t0"], ev["etabar"],
+                   ev["e"], pl=ev["e"] / 3, max_steps=max_steps, tag=tag)
+
+
+def run_batched(V, evs, tab, precision, freeze, max_steps=None, autotune_steps=0, poll_steps=32):
+    """All events resident in one batch until each stops; returns per-event
+    stop step, stop time, and the batch (events still resident)."""
+    budget = sum(V.batch_event_bytes(ev["nx"], ev["ny"], ev["nz"]) for ev in evs) + (64 << 20)
+    B = V.Batch(budget_bytes=budget, max_events=len(evs), eos_table=tab, precision=precision,
+                freezeout_T_GeV=freeze, steps_per_graph=16)
+    t0 = time.perf_counter()
+    slots = [admit_event(B, ev, 0 if max_steps is None else max_steps[i], i) for i, ev in enumerate(evs)]
+    tune = None
+    if autotune_steps:
+        best, ms = B.autotune(autotune_steps)
+        tune = {"best": V.batch_variants()[best], "ms_per_step": [round(float(x), 4) for x in ms]}
+    stop_step, stop_time, left = {}, {}, set(slots)
+    while left:
+        B.step(poll_steps)
+        now = time.perf_counter() - t0
+        for st in B.poll():
+            if st["slot"] in left and st["state"] != V.EVENT_LIVE:
+                left.discard(st["slot"])
+                stop_step[st["tag"]] = st["nstep"]
+                stop_time[st["tag"]] = now
+    wall = time.perf_counter() - t0
+    return dict(B=B, slots=slots, wall=wall, stop_step=[stop_step[i] for i in range(len(evs))],
+                stop_time=[stop_time[i] for i in range(len(evs))], autotune=tune)
+
+
+def run_plain(V, ev, steps, tab):
+    """The plain per-event FP64 path: one lattice context, set-up to teardown."""
+    t0 = time.perf_counter()
+    g = V.Vah(ev["nx"], ev["ny"], ev["nz"], ev["dx"], ev["dx"], ev["dz"], ev["dt"], ev["etabar"], ev["t0"],
+              eos_table=tab)
+    n = ev["nx"] * ev["ny"] * ev["nz"]
+    g.set_primitives(ev["e"], np.ones(n), np.zeros(n), np.zeros(n), np.zeros(n), pl=ev["e"] / 3)
+    g.step(steps)
+    g.synchronize()
+    g.close()
+    return time.perf_counter() - t0
+
+
+def split_errors(a, b):
+    """Max and RMS relative error of e and u^mu (relative to u^tau) of a split-
+    precision state a against the FP64 state b."""
+    e = np.abs(a["e"] - b["e"]) / b["e"]
+    u = np.max(np.abs(a["u"] - b["u"]), axis=0) / b["u"][0]
+    return float(e.max()), float(np.sqrt(np.mean(e * e))), float(u.max()), float(np.sqrt(np.mean(u * u)))
+
+
+def bench_c4(V, args):
+    tab = V.eos_lqcd_table(10000, 1e-8, 1e4)
+    evs = c4_events(V, args.events, seed=2024)
+    cells = np.array([ev["nx"] * ev["ny"] for ev in evs])
+    # 1. the batched split-precision form, freeze-out at T < 150 MeV (autotuned)
+    r32 = run_batched(V, evs, tab, V.FP32_SPLIT, T_FREEZE_GEV, autotune_steps=args.autotune_steps)
+    nsteps = r32["stop_step"]
+    # 2. the batched FP64 form for the same step counts (freeze-out off)
+    r64 = run_batched(V, evs, tab, V.FP64, 0.0, max_steps=nsteps)
+    errs = np.array([split_errors(r32["B"].state(s32, ("e", "u")), r64["B"].state(s64, ("e", "u")))
+                     for s32, s64 in zip(r32["slots"], r64["slots"])])
+    r32["B"].close()
+    r64["B"].close()
+    # 3. the plain per-event FP64 path, one event after the other, same step counts
+    t_plain = [run_plain(V, ev, k, tab) for ev, k in zip(evs, nsteps)]
+    cell_updates = float(np.sum(cells * np.array(nsteps)))
+    n = len(evs)
+    lat = np.array(r32["stop_time"])
+    line = {
+        "metric": "events/s",
+        "value": n / r32["wall"],
+        "unit": "events/s",
+        "n_gpus": 1,
+        "steps": int(np.max(nsteps)),
+        "warmup": 0,
+        "ms_per_step": None,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32/f64",
+        "data": "synthetic",
+        "config": {"workload": f"{n} boost-invariant MC-Glauber Pb+Pb events, side U{{101..261}}, dx U[0.08,0.12], "
+                               "dtau=0.2dx, b U[0,12] fm, tau0=0.5, eta/s=0.16, LQCD-like EOS table, "
+                               "each run to max T < 150 MeV, one batch, split precision",
+                   "events": n, "cells_total": int(cells.sum()), "steps_min_max": [int(min(nsteps)), int(max(nsteps))],
+                   "cell_updates": cell_updates, "autotune": r32["autotune"]},
+        "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                      "note": "all events admitted at t=0; latency = time to the poll that saw it frozen"},
+        "batched_fp32_split": {"events_per_s": n / r32["wall"], "wall_s": r32["wall"],
+                               "cell_updates_per_s": cell_updates / r32["wall"]},
+        "batched_fp64": {"events_per_s": n / r64["wall"], "wall_s": r64["wall"],
+                         "cell_updates_per_s": cell_updates / r64["wall"]},
+        "plain_fp64_per_event": {"events_per_s": n / sum(t_plain), "wall_s": sum(t_plain),
+                                 "cell_updates_per_s": cell_updates / sum(t_plain)},
+        "speedup_vs_plain_fp64": sum(t_plain) / r32["wall"],
+        "split_vs_fp64_error": {"e_max": float(errs[:, 0].max()), "e_rms": float(np.sqrt(np.mean(errs[:, 1] ** 2))),
+                                "u_max": float(errs[:, 2].max()), "u_rms": float(np.sqrt(np.mean(errs[:, 3] ** 2)))},
+        "roofline": {"bound": "hbm", "achieved": BYTES_PER_CELL_UPDATE * cell_updates / r32["wall"] / 1e9,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": BYTES_PER_CELL_UPDATE * cell_updates / r32["wall"] / 1e9 / HBM_PEAK_GBS,
+                     "traffic": None, "kernel": "batch stage kernels (whole-run average, FP64 storage)"},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line))
+
+
+def serving_requests(V, n, seed):
+    """Request mix: 80% boost-invariant L x L (L ~ U{121..301}), 20% 3+1D
+    L x L x nz (L ~ U{121..201}, nz ~ U{41..81}); dx = deta = 0.1, dtau = 0.02,
+    per-request MC-Glauber seeds.  Arrival times are set per rate."""
+    serve = importlib.import_module("gpu-vah_amd.serve")
+    rng = np.random.default_rng(seed)
+    specs = []
+    for i in range(n):
+        if rng.random() < 0.8:
+            L, nz = int(rng.integers(121, 302)), 1
+        else:
+            L, nz = int(rng.integers(121, 202)), int(rng.integers(41, 82))
+        specs.append((i, L, nz, float(rng.uniform(0.0, 12.0)), seed * 100003 + i))
+    es = parallel_map(lambda sp: mc_event(V, sp[1], sp[2], 0.1, 0.1, sp[3], sp[4]), specs)
+    base = rng.exponential(1.0, size=n).cumsum()
+    reqs = [serve.Request(i, 0.0, L, L, nz, 0.1, 0.1, 0.02, 0.5, 0.16, e) for (i, L, nz, _, _), e in zip(specs, es)]
+    return reqs, base
+
+
+def bench_c5(V, args):
+    serve = importlib.import_module("gpu-vah_amd.serve")
+    tab = V.eos_lqcd_table(10000, 1e-8, 1e4)
+    reqs, base = serving_requests(V, args.events, seed=77)
+    rates = [float(r) for r in args.rates.split(",")]
+    sweeps = []
+    for rate in rates:
+        for r, a in zip(reqs, base / rate):
+            r.arrival, r.admitted, r.finished, r.state = float(a), None, None, -1
+        B = V.Batch(budget_bytes=int(args.budget_gb * (1 << 30)), max_events=256, eos_table=tab,
+                    precision=V.FP32_SPLIT, freezeout_T_GeV=T_FREEZE_GEV, steps_per_graph=16)
+        out = serve.serve(B, reqs, steps_per_poll=16)
+        B.close()
+        out["rate"] = rate
+        out["steps"] = [r.nstep for r in reqs]
+        sweeps.append(out)
+        print(f"rate {rate}/s: {out['events_per_s']:.2f} events/s, p50 {out['p50_latency_s']:.2f} s, "
+              f"p99 {out['p99_latency_s']:.2f} s, peak resident {out['peak_resident']}", file=sys.stderr, flush=True)
+    # the sequential plain FP64 path: measured service time of every request
+    nsteps = sweeps[-1]["steps"]
+    service = [run_plain(V, dict(nx=r.nx, ny=r.ny, nz=r.nz, dx=r.dx, dz=r.dz, dt=r.dt, t0=r.t0, etabar=r.etabar,
+                                 e=r.e), k, tab) for r, k in zip(reqs, nsteps)]
+    for sw in sweeps:
+        lat = serve.fifo_latencies(base / sw["rate"], service)
+        finish = max(a + l for a, l in zip(sorted(base / sw["rate"]), lat))
+        sw["sequential_fp64"] = {"events_per_s": len(reqs) / finish, "p50_latency_s": float(np.percentile(lat, 50)),
+                                 "p99_latency_s": float(np.percentile(lat, 99))}
+        del sw["steps"]
+    top = max(sweeps, key=lambda s: s["events_per_s"])
+    line = {
+        "metric": "events/s",
+        "value": top["events_per_s"],
+        "unit": "events/s",
+        "n_gpus": 1,
+        "steps": None,
+        "warmup": 0,
+        "ms_per_step": None,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32/f64",
+        "data": "synthetic",
+        "config": {"workload": f"serving: {len(reqs)} Poisson requests, 80% boost-invariant L^2 (L U{{121..301}}), "
+                               "20% 3+1D L^2 x nz (L U{121..201}, nz U{41..81}), dx=deta=0.1, dtau=0.02, "
+                               "MC-Glauber per-request seeds, retire at max T < 150 MeV, split precision, "
+                               f"HBM budget {args.budget_gb} GiB",
+                   "rates_per_s": rates},
+        "latency_s": {"p50": top["p50_latency_s"], "p99": top["p99_latency_s"]},
+        "sweep": sweeps,
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line))
+
+
 def measured_traffic(workload):
     """HBM bytes per launch of each stage kernel, as measured with rocprofv3
     FETCH_SIZE / WRITE_SIZE passes (scaled by the copy-kernel factor) and
@@ -113,7 +336,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="c2")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["c4", "c5"], default="c2",
+                    help="c2 (default): one 201x201x101 event; c1: 81x81x1; c4: batched events; c5: serving")
+    ap.add_argument("--events", type=int, default=256, help="c4 / c5: number of events / requests")
+    ap.add_argument("--autotune-steps", type=int, default=4, help="c4: steps per variant trial (0: no autotune)")
+    ap.add_argument("--rates", default="2,4,8,16", help="c5: Poisson arrival rates [1/s] to sweep")
+    ap.add_argument("--budget-gb", type=float, default=32.0, help="c5: HBM admission budget [GiB]")
     ap.add_argument("--flux-mode", choices=["shared", "per_cell"], default="shared")
     ap.add_argument("--precision", choices=["fp64", "fp32_split"], default="fp64",
                     help="fp32_split: stencil, fluxes and sources in float, Newton recovery in double")
@@ -131,6 +359,12 @@ def main():
         return
 
     V = importlib.import_module("gpu-vah_amd")
+    if args.workload == "c4":
+        bench_c4(V, args)
+        return
+    if args.workload == "c5":
+        bench_c5(V, args)
+        return
     w = WORKLOADS[args.workload]
     e = initial_state(V, w)
     n = e.size

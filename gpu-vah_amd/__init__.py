@@ -23,6 +23,9 @@ FIELDS = ("ttt", "ttx", "tty", "ttn", "pl", "pitt", "pitx", "pity", "pitn",
 
 VAH_OK = 0
 VAH_ERR_NAN = -4
+VAH_ERR_BUDGET = -7
+VAH_ERR_FULL = -8
+EVENT_LIVE, EVENT_FROZEN, EVENT_STEP_LIMIT, EVENT_NAN = 0, 1, 2, 3
 EOS_CONFORMAL, EOS_TABLE = 0, 1
 FP64, FP32_SPLIT = 0, 1
 FLUX_SHARED, FLUX_PER_CELL = 0, 1
@@ -59,6 +62,32 @@ class Stats(ctypes.Structure):
 
 class Profile(ctypes.Structure):
     _fields_ = [("launches", ctypes.c_int * 2), ("avg_ms", ctypes.c_double * 2)]
+
+
+class BatchOptions(ctypes.Structure):
+    """vah_batch_options of include/vah_batch.h."""
+    _fields_ = [("eos", ctypes.c_int), ("table", EosTable), ("precision", ctypes.c_int),
+                ("budget_bytes", ctypes.c_size_t), ("max_events", ctypes.c_int),
+                ("freezeout_T_GeV", ctypes.c_double), ("steps_per_graph", ctypes.c_int), ("variant", ctypes.c_int),
+                ("device", ctypes.c_int), ("stream", ctypes.c_void_p)]
+
+
+class BatchEvent(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+                ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("dz", ctypes.c_double), ("dt", ctypes.c_double),
+                ("t0", ctypes.c_double), ("etabar", ctypes.c_double), ("max_steps", ctypes.c_longlong),
+                ("tag", ctypes.c_longlong)]
+
+
+class BatchStatus(ctypes.Structure):
+    _fields_ = [("slot", ctypes.c_int), ("state", ctypes.c_int), ("tag", ctypes.c_longlong),
+                ("nstep", ctypes.c_longlong), ("tau", ctypes.c_double), ("max_T", ctypes.c_double),
+                ("recovery_fallbacks", ctypes.c_longlong), ("nan_cells", ctypes.c_longlong)]
+
+
+class BatchVariant(ctypes.Structure):
+    _fields_ = [("tile_x", ctypes.c_int), ("tile_y", ctypes.c_int), ("blocks_per_cu", ctypes.c_int),
+                ("kz", ctypes.c_int)]
 
 
 class VahError(RuntimeError):
@@ -132,6 +161,29 @@ def lib():
         L.vah_config_load.argtypes = [ctypes.c_char_p, ctypes.POINTER(Config)]
         L.vah_ic_reference.argtypes = [ctypes.POINTER(Config)] + [_dp] * 5
         L.vah_run.argtypes = [ctypes.POINTER(Config), ctypes.c_char_p, ctypes.c_int]
+        # batch engine (include/vah_batch.h)
+        L.vah_batch_default_options.argtypes = [ctypes.POINTER(BatchOptions)]
+        L.vah_batch_default_options.restype = None
+        L.vah_batch_event_bytes.argtypes = [ctypes.c_int] * 3
+        L.vah_batch_event_bytes.restype = ctypes.c_size_t
+        L.vah_batch_create.argtypes = [ctypes.POINTER(BatchOptions), ctypes.POINTER(ctypes.c_void_p)]
+        L.vah_batch_admit.argtypes = [ctypes.c_void_p, ctypes.POINTER(BatchEvent)] + [_dp] * 7 + \
+                                     [ctypes.POINTER(ctypes.c_int)]
+        L.vah_batch_step.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.vah_batch_synchronize.argtypes = [ctypes.c_void_p]
+        L.vah_batch_poll.argtypes = [ctypes.c_void_p, ctypes.POINTER(BatchStatus), ctypes.c_int,
+                                     ctypes.POINTER(ctypes.c_int)]
+        L.vah_batch_get_state.argtypes = [ctypes.c_void_p, ctypes.c_int] + [_dp] * 5
+        L.vah_batch_retire.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.vah_batch_memory.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_size_t),
+                                       ctypes.POINTER(ctypes.c_size_t)]
+        L.vah_batch_variant_count.argtypes = []
+        L.vah_batch_variant_info.argtypes = [ctypes.c_int, ctypes.POINTER(BatchVariant)]
+        L.vah_batch_set_variant.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.vah_batch_get_variant.argtypes = [ctypes.c_void_p]
+        L.vah_batch_autotune.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int), _dp]
+        L.vah_batch_destroy.argtypes = [ctypes.c_void_p]
+        L.vah_batch_destroy.restype = None
         _lib = L
     return _lib
 
@@ -278,3 +330,125 @@ class Vah:
         pr = Profile()
         _check(lib().vah_profile_read(self.h, ctypes.byref(pr)))
         return {"launches": list(pr.launches), "avg_ms": list(pr.avg_ms)}
+
+
+def batch_event_bytes(nx, ny, nz):
+    """Arena bytes one resident event of nx x ny x nz interior cells takes."""
+    return lib().vah_batch_event_bytes(nx, ny, nz)
+
+
+def batch_variants():
+    """The compiled kernel variants: (tile_x, tile_y, blocks_per_cu, kz)."""
+    out = []
+    for i in range(lib().vah_batch_variant_count()):
+        v = BatchVariant()
+        _check(lib().vah_batch_variant_info(i, ctypes.byref(v)))
+        out.append((v.tile_x, v.tile_y, v.blocks_per_cu, v.kz))
+    return out
+
+
+class Batch:
+    """Many resident events advanced together (include/vah_batch.h)."""
+
+    def __init__(self, budget_bytes=8 << 30, max_events=1024, eos_table=None, precision=FP64,
+                 freezeout_T_GeV=0.150, steps_per_graph=16, variant=-1, device=0):
+        L = lib()
+        opt = BatchOptions()
+        L.vah_batch_default_options(ctypes.byref(opt))
+        opt.budget_bytes = int(budget_bytes)
+        opt.max_events = max_events
+        opt.precision = precision
+        opt.freezeout_T_GeV = freezeout_T_GeV
+        opt.steps_per_graph = steps_per_graph
+        opt.variant = variant
+        opt.device = device
+        self._keep = []
+        if eos_table is not None:
+            lne0, h, p, T = eos_table
+            p, pp = _arr(p)
+            T, pT = _arr(T)
+            self._keep += [p, T]
+            opt.eos = EOS_TABLE
+            opt.table = EosTable(len(p), lne0, h, pp, pT)
+        h = ctypes.c_void_p()
+        _check(L.vah_batch_create(ctypes.byref(opt), ctypes.byref(h)))
+        self.h = h
+        self.max_events = max_events
+        self.dims = {}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().vah_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def admit(self, nx, ny, nz, dx, dy, dz, dt, t0, etabar, e, ut=None, ux=None, uy=None, un=None, pl=None, pi=None,
+              max_steps=0, tag=0, raise_on_full=True):
+        """Admit one event; returns its slot, or None when the arena or the
+        slot table is full and raise_on_full is False."""
+        n = nx * ny * nz
+        if ut is None:
+            ut, ux, uy, un = np.ones(n), np.zeros(n), np.zeros(n), np.zeros(n)
+        ev = BatchEvent(nx, ny, nz, dx, dy, dz, dt, t0, etabar, max_steps, tag)
+        args = [_arr(x) for x in (e, pl, ut, ux, uy, un, pi)]
+        slot = ctypes.c_int(-1)
+        rc = lib().vah_batch_admit(self.h, ctypes.byref(ev), *[a[1] for a in args], ctypes.byref(slot))
+        if rc in (VAH_ERR_BUDGET, VAH_ERR_FULL) and not raise_on_full:
+            return None
+        _check(rc)
+        self.dims[slot.value] = (nx, ny, nz)
+        return slot.value
+
+    def step(self, n=1):
+        _check(lib().vah_batch_step(self.h, n))
+
+    def synchronize(self):
+        _check(lib().vah_batch_synchronize(self.h))
+
+    def poll(self):
+        """Status of every resident event (dicts, slot order)."""
+        buf = (BatchStatus * self.max_events)()
+        n = ctypes.c_int()
+        _check(lib().vah_batch_poll(self.h, buf, self.max_events, ctypes.byref(n)))
+        return [{k: getattr(buf[i], k) for k, _ in BatchStatus._fields_} for i in range(n.value)]
+
+    def state(self, slot, fields=("q", "e", "p", "u", "up")):
+        """Fields of a resident event at its current step (only those named)."""
+        nx, ny, nz = self.dims[slot]
+        n = nx * ny * nz
+        shape = {"q": (NQ, n), "e": (n,), "p": (n,), "u": (4, n), "up": (4, n)}
+        out = {k: np.empty(shape[k]) for k in fields}
+        ptrs = [out[k].ctypes.data_as(_dp) if k in out else None for k in ("q", "e", "p", "u", "up")]
+        _check(lib().vah_batch_get_state(self.h, slot, *ptrs))
+        return out
+
+    def retire(self, slot):
+        _check(lib().vah_batch_retire(self.h, slot))
+        self.dims.pop(slot, None)
+
+    def memory(self):
+        used, budget = ctypes.c_size_t(), ctypes.c_size_t()
+        _check(lib().vah_batch_memory(self.h, ctypes.byref(used), ctypes.byref(budget)))
+        return used.value, budget.value
+
+    @property
+    def variant(self):
+        return lib().vah_batch_get_variant(self.h)
+
+    @variant.setter
+    def variant(self, v):
+        _check(lib().vah_batch_set_variant(self.h, v))
+
+    def autotune(self, steps_per_trial=4):
+        """Time every kernel variant on the resident events (real steps);
+        returns (best variant, ms per step of each)."""
+        nv = lib().vah_batch_variant_count()
+        ms = np.zeros(nv)
+        best = ctypes.c_int()
+        _check(lib().vah_batch_autotune(self.h, steps_per_trial, ctypes.byref(best), ms.ctypes.data_as(_dp)))
+        return best.value, ms

@@ -24,15 +24,17 @@
 #include <vector>
 
 #include "../../include/vah.h"
+#include "vah_host_common.h"
 #include "vah_stage.h"
 
 using namespace vah;
 
 namespace {
-
 thread_local std::string g_err;
+}  // namespace
 
-int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+namespace vah_host {
+
 int fail(int code, const char *fmt, ...) {
     char buf[512];
     va_list ap;
@@ -43,46 +45,95 @@ int fail(int code, const char *fmt, ...) {
     return code;
 }
 
-#define HIP_TRY(expr)                                                                      \
-    do {                                                                                   \
-        hipError_t _e = (expr);                                                            \
-        if (_e != hipSuccess) return fail(VAH_ERR_HIP, "%s: %s", #expr, hipGetErrorString(_e)); \
-    } while (0)
+}  // namespace vah_host
+
+using vah_host::fail;
 
 // ---- glibc-compatible powf(x, 0.25f) correction bitmap -------------------
 // Bit `off` is set when the host libm's powf differs from the correctly
 // rounded fourth root for x with bit pattern (124 << 23) + off.  The pattern
 // repeats every 4 binades (see conformal_T in vah_device.h).
-std::once_flag g_fix_once;
-std::vector<uint32_t> g_fix;
+namespace vah_host {
 
-void build_powf_fix() {
-    const uint32_t nbits = 1u << 25;
-    g_fix.assign(nbits / 32, 0u);
-    unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    std::vector<std::thread> th;
-    const uint32_t words = nbits / 32, per = (words + nth - 1) / nth;
-    for (unsigned t = 0; t < nth; ++t) {
-        th.emplace_back([t, per, words] {
-            uint32_t w0 = t * per, w1 = std::min(words, w0 + per);
-            for (uint32_t w = w0; w < w1; ++w) {
-                uint32_t word = 0;
-                for (uint32_t bit = 0; bit < 32; ++bit) {
-                    uint32_t b = (124u << 23) + w * 32 + bit;
-                    float x;
-                    memcpy(&x, &b, 4);
-                    float g = powf(x, 0.25f);
-                    float r = (float)sqrt(sqrt((double)x));
-                    if (g != r) word |= 1u << bit;
+const std::vector<uint32_t> &powf_fix_table() {
+    static std::once_flag once;
+    static std::vector<uint32_t> fix;
+    std::call_once(once, [] {
+        const uint32_t nbits = 1u << 25;
+        fix.assign(nbits / 32, 0u);
+        unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        std::vector<std::thread> th;
+        const uint32_t words = nbits / 32, per = (words + nth - 1) / nth;
+        for (unsigned t = 0; t < nth; ++t) {
+            th.emplace_back([t, per, words] {
+                uint32_t w0 = t * per, w1 = std::min(words, w0 + per);
+                for (uint32_t w = w0; w < w1; ++w) {
+                    uint32_t word = 0;
+                    for (uint32_t bit = 0; bit < 32; ++bit) {
+                        uint32_t b = (124u << 23) + w * 32 + bit;
+                        float x;
+                        memcpy(&x, &b, 4);
+                        float g = powf(x, 0.25f);
+                        float r = (float)sqrt(sqrt((double)x));
+                        if (g != r) word |= 1u << bit;
+                    }
+                    fix[w] = word;
                 }
-                g_fix[w] = word;
-            }
-        });
-    }
-    for (auto &x : th) x.join();
+            });
+        }
+        for (auto &x : th) x.join();
+    });
+    return fix;
 }
 
-}  // namespace
+int make_device_eos(int kind, const vah_eos_table &T, hipStream_t stream, Eos *out, uint32_t **fix_dev,
+                    double **tab_dev, HostEos *host, std::string *err) {
+    memset(out, 0, sizeof *out);
+    *fix_dev = nullptr;
+    *tab_dev = nullptr;
+    host->kind = kind;
+    out->kind = kind == VAH_EOS_TABLE ? 1 : 0;
+    if (kind == VAH_EOS_CONFORMAL) {
+        const std::vector<uint32_t> &fix = powf_fix_table();
+        if (hipMalloc(fix_dev, fix.size() * sizeof(uint32_t)) != hipSuccess) {
+            *err = "powf table allocation";
+            return VAH_ERR_NOMEM;
+        }
+        if (hipMemcpyAsync(*fix_dev, fix.data(), fix.size() * sizeof(uint32_t), hipMemcpyHostToDevice, stream) !=
+                hipSuccess ||
+            hipStreamSynchronize(stream) != hipSuccess) {
+            *err = "powf table upload";
+            return VAH_ERR_HIP;
+        }
+        out->powf_fix = *fix_dev;
+        return VAH_OK;
+    }
+    if (hipMalloc(tab_dev, sizeof(double) * 2 * T.n) != hipSuccess) {
+        *err = "EOS table allocation";
+        return VAH_ERR_NOMEM;
+    }
+    std::vector<double> tab(2 * (size_t)T.n);
+    memcpy(tab.data(), T.p, sizeof(double) * T.n);
+    memcpy(tab.data() + T.n, T.T, sizeof(double) * T.n);
+    if (hipMemcpy(*tab_dev, tab.data(), sizeof(double) * 2 * T.n, hipMemcpyHostToDevice) != hipSuccess) {
+        *err = "EOS table upload";
+        return VAH_ERR_HIP;
+    }
+    out->n = T.n;
+    out->lne0 = T.lne0;
+    out->inv_h = 1.0 / T.h;
+    out->e_first = exp(T.lne0);
+    out->e_last = exp(T.lne0 + (T.n - 1) / (1.0 / T.h));
+    out->p = *tab_dev;
+    out->T = *tab_dev + T.n;
+    host->n = T.n;
+    host->lne0 = T.lne0;
+    host->h = T.h;
+    host->p.assign(T.p, T.p + T.n);
+    return VAH_OK;
+}
+
+}  // namespace vah_host
 
 struct vah_ctx {
     vah_lattice lat;
@@ -105,6 +156,8 @@ struct vah_ctx {
     uint32_t *powf_fix;
     double *eos_dev;       // p then T
     Eos eos;
+    vah_host::HostEos heos;
+    vah_host::Layout lay;
     long long nstep;
     bool have_state;
     // launch geometry
@@ -217,32 +270,9 @@ int fill_ghosts(vah_ctx *c) {
     return VAH_OK;
 }
 
-inline long comp_index(const vah_ctx *c, long m) {
-    int i = (int)(m % c->lat.nx);
-    int j = (int)((m / c->lat.nx) % c->lat.ny);
-    int k = (int)(m / ((long)c->lat.nx * c->lat.ny));
-    return (long)(k + 2) * c->ncx * c->ncy + (long)(j + 2) * c->ncx + (i + 2);
-}
-
-// host EOS pressure for initial states (same lookup as the device)
-double host_eos_p(const vah_ctx *c, const std::vector<double> &tab, double e) {
-    if (c->opt.eos == VAH_EOS_CONFORMAL) return e / 3;
-    const vah_eos_table &T = c->opt.table;
-    double f = (log(e) - T.lne0) * (1.0 / T.h);
-    int last = T.n - 1;
-    if (!(f >= 0.0)) return tab[0] * (e / exp(T.lne0));
-    int i = (int)f;
-    if (i >= last) return tab[last] * (e / exp(T.lne0 + last / (1.0 / T.h)));
-    double w = f - (double)i;
-    return tab[i] + w * (tab[i + 1] - tab[i]);
-}
-
 // copy interior host arrays into the padded device layout of one slab
 int upload_fields(vah_ctx *c, double *dst_slab, int nfields, const double *src, std::vector<double> &stage_buf) {
-    long n = (long)c->lat.nx * c->lat.ny * c->lat.nz;
-    stage_buf.assign((size_t)nfields * c->fs, 0.0);
-    for (int f = 0; f < nfields; ++f)
-        for (long m = 0; m < n; ++m) stage_buf[(size_t)f * c->fs + comp_index(c, m)] = src[(size_t)f * n + m];
+    c->lay.pad(nfields, src, stage_buf);
     HIP_TRY(hipMemcpyAsync(dst_slab, stage_buf.data(), sizeof(double) * nfields * c->fs, hipMemcpyHostToDevice,
                            c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
@@ -250,12 +280,10 @@ int upload_fields(vah_ctx *c, double *dst_slab, int nfields, const double *src, 
 }
 
 int download_fields(vah_ctx *c, const double *src_slab, int nfields, double *dst) {
-    long n = (long)c->lat.nx * c->lat.ny * c->lat.nz;
     std::vector<double> buf((size_t)nfields * c->fs);
     HIP_TRY(hipMemcpyAsync(buf.data(), src_slab, sizeof(double) * nfields * c->fs, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
-    for (int f = 0; f < nfields; ++f)
-        for (long m = 0; m < n; ++m) dst[(size_t)f * n + m] = buf[(size_t)f * c->fs + comp_index(c, m)];
+    c->lay.unpad(nfields, buf.data(), dst);
     return VAH_OK;
 }
 
@@ -268,7 +296,7 @@ __global__ void reduce_kernel(const double *q, const double *e, const double *u,
         int ii = (int)(m % nx);
         int jj = (int)((m / nx) % ny);
         int kk = (int)(m / ((long)nx * ny));
-        long s = (long)(kk + 2) * sz + (long)(jj + 2) * sy + (ii + 2);
+        long s = (long)plane_of(kk, nz > 1) * sz + (long)(jj + 2) * sy + (ii + 2);
         double ev = e[s];
         mx_e = fmax(mx_e, ev);
         mx_T = fmax(mx_T, eos_T(eos, ev));
@@ -345,11 +373,12 @@ int vah_create(const vah_lattice *lat, const vah_hydro *hydro, const vah_options
     c->hydro = *hydro;
     c->opt = opt;
     c->device = opt.device;
-    c->ncx = lat->nx + 4;
-    c->ncy = lat->ny + 4;
-    c->ncz = lat->nz + 4;
-    c->ncell = (long)c->ncx * c->ncy * c->ncz;
-    c->fs = (c->ncell + 63) / 64 * 64;
+    c->lay = vah_host::Layout(lat->nx, lat->ny, lat->nz);
+    c->ncx = c->lay.ncx;
+    c->ncy = c->lay.ncy;
+    c->ncz = c->lay.ncz;
+    c->ncell = c->lay.ncell;
+    c->fs = c->lay.fs;
     c->qi = 0;
     c->qs = 1;
     c->qo = 2;
@@ -386,39 +415,19 @@ int vah_create(const vah_lattice *lat, const vah_hydro *hydro, const vah_options
         hipMemsetAsync(c->counters, 0, 4 * sizeof(unsigned), c->stream) != hipSuccess)
         return cleanup(fail(VAH_ERR_NOMEM, "vah_create: counters"));
 
-    memset(&c->eos, 0, sizeof c->eos);
-    c->eos.kind = opt.eos == VAH_EOS_TABLE ? 1 : 0;
-    if (opt.eos == VAH_EOS_CONFORMAL) {
-        std::call_once(g_fix_once, build_powf_fix);
-        if (hipMalloc(&c->powf_fix, g_fix.size() * sizeof(uint32_t)) != hipSuccess ||
-            hipMemcpyAsync(c->powf_fix, g_fix.data(), g_fix.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
-                           c->stream) != hipSuccess)
-            return cleanup(fail(VAH_ERR_NOMEM, "vah_create: powf table"));
-        c->eos.powf_fix = c->powf_fix;
-    } else {
-        const vah_eos_table &T = opt.table;
-        if (hipMalloc(&c->eos_dev, sizeof(double) * 2 * T.n) != hipSuccess)
-            return cleanup(fail(VAH_ERR_NOMEM, "vah_create: EOS table"));
-        std::vector<double> tab(2 * (size_t)T.n);
-        memcpy(tab.data(), T.p, sizeof(double) * T.n);
-        memcpy(tab.data() + T.n, T.T, sizeof(double) * T.n);
-        if (hipMemcpy(c->eos_dev, tab.data(), sizeof(double) * 2 * T.n, hipMemcpyHostToDevice) != hipSuccess)
-            return cleanup(fail(VAH_ERR_HIP, "vah_create: EOS upload"));
-        c->eos.n = T.n;
-        c->eos.lne0 = T.lne0;
-        c->eos.inv_h = 1.0 / T.h;
-        c->eos.e_first = exp(T.lne0);
-        c->eos.e_last = exp(T.lne0 + (T.n - 1) / (1.0 / T.h));
-        c->eos.p = c->eos_dev;
-        c->eos.T = c->eos_dev + T.n;
-        // keep a host copy of the pressure nodes for initial states
+    {
+        std::string err;
+        int rc = vah_host::make_device_eos(opt.eos, opt.table, c->stream, &c->eos, &c->powf_fix, &c->eos_dev,
+                                           &c->heos, &err);
+        if (rc != VAH_OK) return cleanup(fail(rc, "vah_create: %s", err.c_str()));
+        // the table arrays belong to the caller: only the copies are kept
         c->opt.table.p = nullptr;
         c->opt.table.T = nullptr;
     }
 
     // launch geometry
-    c->tiles_x = (lat->nx + TX - 1) / TX;
-    c->tiles_y = (lat->ny + TY - 1) / TY;
+    c->tiles_x = (lat->nx + kTileX - 1) / kTileX;
+    c->tiles_y = (lat->ny + kTileY - 1) / kTileY;
     c->kz = lat->nz > 1 ? std::min(8, lat->nz) : 1;
     c->zchunks = (lat->nz + c->kz - 1) / c->kz;
     c->nblocks = c->tiles_x * c->tiles_y * c->zchunks;
@@ -451,39 +460,9 @@ int vah_set_primitives(vah_ctx *c, const double *e, const double *pl, const doub
     if (!c || !e || !ut || !ux || !uy || !un) return fail(VAH_ERR_ARG, "vah_set_primitives: NULL argument");
     HIP_TRY(hipSetDevice(c->device));
     const long n = (long)c->lat.nx * c->lat.ny * c->lat.nz;
-    const double t = c->hydro.t0;
-    std::vector<double> ptab;
-    if (c->opt.eos == VAH_EOS_TABLE) {
-        ptab.resize(c->eos.n);
-        HIP_TRY(hipMemcpy(ptab.data(), c->eos_dev, sizeof(double) * c->eos.n, hipMemcpyDeviceToHost));
-    }
     std::vector<double> q((size_t)NQ * n), ev(n), pv(n), u((size_t)4 * n);
-    for (long m = 0; m < n; ++m) {
-        // setConservedVariables (DynamicalVariables.cpp:197-287)
-        double eeq = e[m];
-        double peq = host_eos_p(c, ptab, eeq);
-        double vt = ut[m], vx = ux[m], vy = uy[m], vn = un[m];
-        double pis[10];
-        for (int f = 0; f < 10; ++f) pis[f] = pi ? pi[(size_t)f * n + m] : 0.0;
-        double plv = pl ? pl[m] : 0.0154483 * eeq;
-        double pt = 0.5 * (eeq - plv);
-        double DP = plv - pt;
-        double F = 1 + (vx * vx + vy * vy);
-        double Ltt = DP * t * t * vn * vn / F;
-        double Ltn = DP * vt * vn / F;
-        q[(size_t)TTT * n + m] = (eeq + pt) * vt * vt - pt + Ltt + pis[0];
-        q[(size_t)TTX * n + m] = (eeq + pt) * vt * vx + pis[1];
-        q[(size_t)TTY * n + m] = (eeq + pt) * vt * vy + pis[2];
-        q[(size_t)TTN * n + m] = (eeq + pt) * vt * vn + Ltn + pis[3];
-        q[(size_t)PL * n + m] = plv;
-        for (int f = 0; f < 10; ++f) q[(size_t)(PITT + f) * n + m] = pis[f];
-        ev[m] = eeq;
-        pv[m] = peq;
-        u[m] = vt;
-        u[(size_t)n + m] = vx;
-        u[(size_t)2 * n + m] = vy;
-        u[(size_t)3 * n + m] = vn;
-    }
+    vah_host::prims_to_conserved(c->hydro.t0, c->heos, n, e, pl, ut, ux, uy, un, pi, q.data(), ev.data(), pv.data(),
+                                 u.data());
     return vah_set_state(c, 0, q.data(), ev.data(), pv.data(), u.data(), u.data());
 }
 

@@ -25,6 +25,10 @@ enum { TTT, TTX, TTY, TTN, PL, PITT, PITX, PITY, PITN, PIXX, PIXY, PIXN, PIYY, P
 
 constexpr double kThird = 1.0 / 3.0;
 
+// Storage plane of interior eta index k: 3+1D lattices carry 2 ghost planes
+// on each side, a boost-invariant (one-plane) lattice stores its plane alone.
+__host__ __device__ constexpr int plane_of(int k, bool d3) { return d3 ? k + 2 : 0; }
+
 // ---------------------------------------------------------------------------
 // Reciprocal, division and reciprocal square root from the hardware
 // approximations (v_rcp_f64, v_rsq_f64) refined by two Newton-Raphson steps,

@@ -34,9 +34,16 @@ namespace vah {
 #define VAH_BLOCKS_3D 2
 #endif
 
-constexpr int TX = 16;
-constexpr int TY = 15;
+// default tile: 16 x 15 cells (x faces 17 x 15, y faces 16 x 16 fill 255 / 256 lanes)
+constexpr int kTileX = 16;
+constexpr int kTileY = 15;
 constexpr int NTHREADS = 256;
+
+// a tile shape fits one workgroup when its cells and both face sets do
+template <int TX, int TY>
+constexpr bool tile_fits() {
+    return TX * TY <= NTHREADS && (TX + 1) * TY <= NTHREADS && TX * (TY + 1) <= NTHREADS;
+}
 
 struct StageParams {
     const double *qin;    // 15-field slab the stage reads
@@ -340,9 +347,11 @@ __device__ __forceinline__ void count_flags(const StageParams &P, bool nan, bool
 // One tile of one stage.  Hs holds the x / y face fluxes of the tile
 // (SHARED) or this lane's forward face (reference split); Hz is the eta face
 // carried from the plane below, one slot per lane that only its own lane
-// touches.  R is the arithmetic type of the stencil, fluxes and sources.
-template <bool D3, bool STAGE2, bool SHARED, typename R>
+// touches.  R is the arithmetic type of the stencil, fluxes and sources; the
+// tile is TX x TY cells (P.tiles_x / P.tiles_y must be counted with them).
+template <bool D3, bool STAGE2, bool SHARED, typename R, int TX = kTileX, int TY = kTileY>
 __device__ __forceinline__ void stage_tile(const StageParams &P, int tile, R (*Hs)[NTHREADS], R (*Hz)[NTHREADS]) {
+    static_assert(tile_fits<TX, TY>(), "tile does not fit one workgroup");
     const int tpl = P.tiles_x * P.tiles_y;
     const int zc = tile / tpl;
     const int txy = tile - zc * tpl;
@@ -363,7 +372,7 @@ __device__ __forceinline__ void stage_tile(const StageParams &P, int tile, R (*H
     const bool cell_ok = cj < TY && ii < P.nx && jj < P.ny;
 
     for (int kk = k0; kk < k1; ++kk) {
-        const long plane = (long)(kk + 2) * sz;
+        const long plane = (long)plane_of(kk, D3) * sz;
         const long s = plane + (long)(jj + 2) * sy + (ii + 2);
         bool nan = false, fallback = false;
         double acc[NQ];
@@ -484,7 +493,7 @@ __global__ __launch_bounds__(NTHREADS, D3 ? VAH_BLOCKS_3D : VAH_BLOCKS_2D) void 
 
 // Zero-gradient ghost copies of an initial state: each boundary interior cell
 // writes its face ghosts (every read of the stencils touches only those).
-__global__ void ghost_fill_kernel(double *q, double *e, double *p, double *u, long fs, int nx, int ny, int nz,
+static __global__ void ghost_fill_kernel(double *q, double *e, double *p, double *u, long fs, int nx, int ny, int nz,
                                   long sy, long sz, int d3) {
     long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
     long n = (long)nx * ny * nz;
@@ -492,7 +501,7 @@ __global__ void ghost_fill_kernel(double *q, double *e, double *p, double *u, lo
     int ii = (int)(m % nx);
     int jj = (int)((m / nx) % ny);
     int kk = (int)(m / ((long)nx * ny));
-    long s = (long)(kk + 2) * sz + (long)(jj + 2) * sy + (ii + 2);
+    long s = (long)plane_of(kk, d3 != 0) * sz + (long)(jj + 2) * sy + (ii + 2);
     auto copy = [&](long d) {
         for (int f = 0; f < NQ; ++f) q[f * fs + d] = q[f * fs + s];
         e[d] = e[s];

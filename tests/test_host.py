@@ -14,7 +14,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def declared_functions():
     names = set()
-    for h in ("vah.h", "vah_host.h"):
+    for h in ("vah.h", "vah_host.h", "vah_batch.h"):
         txt = open(os.path.join(ROOT, "include", h)).read()
         txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
         names |= set(re.findall(r"\b(vah_[a-z0-9_]+)\s*\(", txt))
@@ -24,7 +24,7 @@ def declared_functions():
 def test_library_exports_every_declared_entry_point():
     L = vah().lib()
     names = declared_functions()
-    assert len(names) >= 20
+    assert len(names) >= 35
     missing = [n for n in names if not hasattr(L, n)]
     assert not missing, missing
 
@@ -136,3 +136,22 @@ def test_reference_initial_conditions(tmp_path):
     c.initialConditionType = 7
     with pytest.raises(V.VahError):
         V.ic_reference(c)
+
+
+def test_batch_event_bytes_and_variants():
+    """Arena accounting of the batch engine: 61 padded fields per event, one
+    stored plane for boost-invariant lattices, 2 ghost planes per side in eta
+    otherwise; every compiled variant's tile fits one 256-lane workgroup."""
+    V = vah()
+    def expect(nx, ny, nz):
+        ncz = nz + 4 if nz > 1 else 1
+        fs = -(-((nx + 4) * (ny + 4) * ncz) // 64) * 64
+        return -(-(8 * fs * 61) // 256) * 256
+    for dims in ((101, 101, 1), (261, 131, 1), (121, 121, 41), (1, 1, 1), (7, 3, 2)):
+        assert V.batch_event_bytes(*dims) == expect(*dims), dims
+    assert V.batch_event_bytes(0, 5, 5) == 0
+    vs = V.batch_variants()
+    assert len(vs) >= 4 and vs[0] == (16, 15, 2, 8)
+    for tx, ty, b, kz in vs:
+        assert tx * ty <= 256 and (tx + 1) * ty <= 256 and tx * (ty + 1) <= 256
+        assert b in (1, 2, 3) and kz in (4, 8)

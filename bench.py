@@ -133,7 +133,7 @@ def run_batched(V, evs, tab, precision, freeze, max_steps=None, autotune_steps=0
     if autotune_steps:
         best, ms = B.autotune(autotune_steps)
         tune = {"best": V.batch_variants()[best], "ms_per_step": [round(float(x), 4) for x in ms]}
-    stop_step, stop_time, left = {}, {}, set(slots)
+    stop_step, stop_time, stop_state, left = {}, {}, {}, set(slots)
     while left:
         B.step(poll_steps)
         now = time.perf_counter() - t0
@@ -142,22 +142,32 @@ def run_batched(V, evs, tab, precision, freeze, max_steps=None, autotune_steps=0
                 left.discard(st["slot"])
                 stop_step[st["tag"]] = st["nstep"]
                 stop_time[st["tag"]] = now
+                stop_state[st["tag"]] = st["state"]
     wall = time.perf_counter() - t0
-    return dict(B=B, slots=slots, wall=wall, stop_step=[stop_step[i] for i in range(len(evs))],
-                stop_time=[stop_time[i] for i in range(len(evs))], autotune=tune)
+    n = len(evs)
+    return dict(B=B, slots=slots, wall=wall, stop_step=[stop_step[i] for i in range(n)],
+                stop_time=[stop_time[i] for i in range(n)], stop_state=[stop_state[i] for i in range(n)],
+                autotune=tune)
 
 
 def run_plain(V, ev, steps, tab):
-    """The plain per-event FP64 path: one lattice context, set-up to teardown."""
+    """The plain per-event FP64 path: one lattice context, set-up to teardown.
+    Returns the wall time and whether the run stayed free of NaN."""
     t0 = time.perf_counter()
     g = V.Vah(ev["nx"], ev["ny"], ev["nz"], ev["dx"], ev["dx"], ev["dz"], ev["dt"], ev["etabar"], ev["t0"],
               eos_table=tab)
     n = ev["nx"] * ev["ny"] * ev["nz"]
     g.set_primitives(ev["e"], np.ones(n), np.zeros(n), np.zeros(n), np.zeros(n), pl=ev["e"] / 3)
     g.step(steps)
-    g.synchronize()
+    clean = True
+    try:
+        g.synchronize()
+    except V.VahError as err:
+        if err.code != V.VAH_ERR_NAN:
+            raise
+        clean = False
     g.close()
-    return time.perf_counter() - t0
+    return time.perf_counter() - t0, clean
 
 
 def split_errors(a, b):
@@ -173,8 +183,14 @@ def bench_c4(V, args):
     evs = c4_events(V, args.events, seed=2024)
     cells = np.array([ev["nx"] * ev["ny"] for ev in evs])
     # 1. the batched split-precision form, freeze-out at T < 150 MeV (autotuned)
-    r32 = run_batched(V, evs, tab, V.FP32_SPLIT, T_FREEZE_GEV, autotune_steps=args.autotune_steps)
+    r32 = run_batched(V, evs, tab, V.FP32_SPLIT, T_FREEZE_GEV, autotune_steps=args.autotune_steps,
+                      max_steps=[args.max_steps] * len(evs))
     nsteps = r32["stop_step"]
+    states = np.array(r32["stop_state"])
+    for i in np.nonzero(states != V.EVENT_FROZEN)[0]:
+        ev = evs[i]
+        print(f"event {i}: L={ev['nx']} dx={ev['dx']:.3f} b={ev['b']:.1f} stopped with state {states[i]} "
+              f"at step {nsteps[i]}", file=sys.stderr, flush=True)
     # 2. the batched FP64 form for the same step counts (freeze-out off)
     r64 = run_batched(V, evs, tab, V.FP64, 0.0, max_steps=nsteps)
     errs = np.array([split_errors(r32["B"].state(s32, ("e", "u")), r64["B"].state(s64, ("e", "u")))
@@ -182,7 +198,8 @@ def bench_c4(V, args):
     r32["B"].close()
     r64["B"].close()
     # 3. the plain per-event FP64 path, one event after the other, same step counts
-    t_plain = [run_plain(V, ev, k, tab) for ev, k in zip(evs, nsteps)]
+    plain = [run_plain(V, ev, k, tab) for ev, k in zip(evs, nsteps)]
+    t_plain = [p[0] for p in plain]
     cell_updates = float(np.sum(cells * np.array(nsteps)))
     n = len(evs)
     lat = np.array(r32["stop_time"])
@@ -203,7 +220,11 @@ def bench_c4(V, args):
                                "dtau=0.2dx, b U[0,12] fm, tau0=0.5, eta/s=0.16, LQCD-like EOS table, "
                                "each run to max T < 150 MeV, one batch, split precision",
                    "events": n, "cells_total": int(cells.sum()), "steps_min_max": [int(min(nsteps)), int(max(nsteps))],
-                   "cell_updates": cell_updates, "autotune": r32["autotune"]},
+                   "cell_updates": cell_updates, "autotune": r32["autotune"],
+                   "stopped_frozen": int(np.sum(states == V.EVENT_FROZEN)),
+                   "stopped_nan": int(np.sum(states == V.EVENT_NAN)),
+                   "stopped_step_limit": int(np.sum(states == V.EVENT_STEP_LIMIT)),
+                   "plain_fp64_runs_with_nan": int(sum(not p[1] for p in plain))},
         "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                       "note": "all events admitted at t=0; latency = time to the poll that saw it frozen"},
         "batched_fp32_split": {"events_per_s": n / r32["wall"], "wall_s": r32["wall"],
@@ -340,6 +361,7 @@ def main():
                     help="c2 (default): one 201x201x101 event; c1: 81x81x1; c4: batched events; c5: serving")
     ap.add_argument("--events", type=int, default=256, help="c4 / c5: number of events / requests")
     ap.add_argument("--autotune-steps", type=int, default=4, help="c4: steps per variant trial (0: no autotune)")
+    ap.add_argument("--max-steps", type=int, default=5000, help="c4: step limit per event")
     ap.add_argument("--rates", default="2,4,8,16", help="c5: Poisson arrival rates [1/s] to sweep")
     ap.add_argument("--budget-gb", type=float, default=32.0, help="c5: HBM admission budget [GiB]")
     ap.add_argument("--flux-mode", choices=["shared", "per_cell"], default="shared")

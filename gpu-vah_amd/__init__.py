@@ -26,7 +26,7 @@ VAH_ERR_NAN = -4
 VAH_ERR_BUDGET = -7
 VAH_ERR_FULL = -8
 EVENT_LIVE, EVENT_FROZEN, EVENT_STEP_LIMIT, EVENT_NAN = 0, 1, 2, 3
-EOS_CONFORMAL, EOS_TABLE = 0, 1
+EOS_CONFORMAL, EOS_TABLE, EOS_WB = 0, 1, 2
 FP64, FP32_SPLIT = 0, 1
 FLUX_SHARED, FLUX_PER_CELL = 0, 1
 
@@ -256,7 +256,9 @@ class Vah:
     """One lattice evolved on the GPU through the C ABI."""
 
     def __init__(self, nx, ny, nz, dx, dy, dz, dt, etabar, t0, eos_table=None, flux_mode=FLUX_SHARED,
-                 precision=FP64, device=0, nt=0, freezeout_T_GeV=0.155):
+                 precision=FP64, device=0, nt=0, freezeout_T_GeV=0.155, eos_kind=None):
+        """eos_table (lne0, h, p, T) selects the tabulated EOS; eos_kind=EOS_WB the
+        reference's Wuppertal-Budapest fits; neither: the conformal EOS."""
         L = lib()
         self.nx, self.ny, self.nz = nx, ny, nz
         self.n = nx * ny * nz
@@ -275,6 +277,8 @@ class Vah:
             self._keep += [p, T]
             opt.eos = EOS_TABLE
             opt.table = EosTable(len(p), lne0, h, pp, pT)
+        if eos_kind is not None:
+            opt.eos = eos_kind
         h = ctypes.c_void_p()
         _check(L.vah_create(ctypes.byref(lat), ctypes.byref(hyd), ctypes.byref(opt), ctypes.byref(h)))
         self.h = h
@@ -351,7 +355,7 @@ class Batch:
     """Many resident events advanced together (include/vah_batch.h)."""
 
     def __init__(self, budget_bytes=8 << 30, max_events=1024, eos_table=None, precision=FP64,
-                 freezeout_T_GeV=0.150, steps_per_graph=16, variant=-1, device=0):
+                 freezeout_T_GeV=0.150, steps_per_graph=16, variant=-1, device=0, eos_kind=None):
         L = lib()
         opt = BatchOptions()
         L.vah_batch_default_options(ctypes.byref(opt))
@@ -370,6 +374,8 @@ class Batch:
             self._keep += [p, T]
             opt.eos = EOS_TABLE
             opt.table = EosTable(len(p), lne0, h, pp, pT)
+        if eos_kind is not None:
+            opt.eos = eos_kind
         h = ctypes.c_void_p()
         _check(L.vah_batch_create(ctypes.byref(opt), ctypes.byref(h)))
         self.h = h

@@ -361,7 +361,7 @@ int vah_batch_create(const vah_batch_options *opt_in, vah_batch **out) {
         return fail(VAH_ERR_ARG, "vah_batch_create: unknown variant %d", opt.variant);
     if (opt.eos == VAH_EOS_TABLE && (opt.table.n < 2 || !opt.table.p || !opt.table.T || !(opt.table.h > 0)))
         return fail(VAH_ERR_ARG, "vah_batch_create: EOS table needs n >= 2 nodes, h > 0 and p, T arrays");
-    if (opt.eos != VAH_EOS_CONFORMAL && opt.eos != VAH_EOS_TABLE)
+    if (opt.eos != VAH_EOS_CONFORMAL && opt.eos != VAH_EOS_TABLE && opt.eos != VAH_EOS_WB)
         return fail(VAH_ERR_ARG, "vah_batch_create: unknown EOS kind %d", opt.eos);
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= opt.device || opt.device < 0)
@@ -433,6 +433,8 @@ int vah_batch_admit(vah_batch *b, const vah_batch_event *d, const double *e, con
             break;
         }
     if (s < 0) return fail(VAH_ERR_FULL, "vah_batch_admit: all %d event slots are taken", b->max_events);
+    if ((double)vah_host::Layout(d->nx, d->ny, d->nz).fs * NQ * 8 >= 4294967296.0)
+        return fail(VAH_ERR_ARG, "vah_batch_admit: a 15-field set of this lattice exceeds 4 GB");
     const size_t bytes = event_bytes(d->nx, d->ny, d->nz);
     size_t off = 0;
     if (!arena_alloc(b, bytes, &off))

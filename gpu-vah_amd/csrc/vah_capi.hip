@@ -92,7 +92,8 @@ int make_device_eos(int kind, const vah_eos_table &T, hipStream_t stream, Eos *o
     *fix_dev = nullptr;
     *tab_dev = nullptr;
     host->kind = kind;
-    out->kind = kind == VAH_EOS_TABLE ? 1 : 0;
+    out->kind = kind;
+    if (kind == VAH_EOS_WB) return VAH_OK;  // closed-form fits, nothing to upload
     if (kind == VAH_EOS_CONFORMAL) {
         const std::vector<uint32_t> &fix = powf_fix_table();
         if (hipMalloc(fix_dev, fix.size() * sizeof(uint32_t)) != hipSuccess) {
@@ -360,7 +361,7 @@ int vah_create(const vah_lattice *lat, const vah_hydro *hydro, const vah_options
         return fail(VAH_ERR_ARG, "vah_create: unknown flux mode %d", opt.flux_mode);
     if (opt.eos == VAH_EOS_TABLE && (opt.table.n < 2 || !opt.table.p || !opt.table.T || !(opt.table.h > 0)))
         return fail(VAH_ERR_ARG, "vah_create: EOS table needs n >= 2 nodes, h > 0 and p, T arrays");
-    if (opt.eos != VAH_EOS_CONFORMAL && opt.eos != VAH_EOS_TABLE)
+    if (opt.eos != VAH_EOS_CONFORMAL && opt.eos != VAH_EOS_TABLE && opt.eos != VAH_EOS_WB)
         return fail(VAH_ERR_ARG, "vah_create: unknown EOS kind %d", opt.eos);
 
     int ndev = 0;
@@ -368,6 +369,13 @@ int vah_create(const vah_lattice *lat, const vah_hydro *hydro, const vah_options
         return fail(VAH_ERR_DEVICE, "vah_create: HIP device %d not available (%d found)", opt.device, ndev);
     HIP_TRY(hipSetDevice(opt.device));
 
+    {
+        // each 15-field set is addressed through one buffer resource (< 4 GB)
+        vah_host::Layout L(lat->nx, lat->ny, lat->nz);
+        if ((double)L.fs * NQ * 8 >= 4294967296.0)
+            return fail(VAH_ERR_ARG, "vah_create: %ld padded cells per field exceed one context's 4 GB field-set "
+                        "limit; split the lattice in eta slabs", L.fs);
+    }
     vah_ctx *c = new vah_ctx();
     c->lat = *lat;
     c->hydro = *hydro;

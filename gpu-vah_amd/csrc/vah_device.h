@@ -18,6 +18,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "vah_eos_wb.h"
+
 namespace vah {
 
 constexpr int NQ = 15;
@@ -68,7 +70,7 @@ __device__ __forceinline__ double rsqrt(double x) {
 // Equation of state
 // ---------------------------------------------------------------------------
 struct Eos {
-    int kind;              // 0 conformal, 1 table
+    int kind;              // 0 conformal, 1 table, 2 Wuppertal-Budapest fits
     int n;                 // table nodes
     double lne0, inv_h;    // node i at ln e = lne0 + i / inv_h
     double e_first, e_last;
@@ -125,6 +127,7 @@ __device__ __forceinline__ void table_lookup(const Eos &E, double e, double *p, 
 
 __device__ __forceinline__ double eos_p(const Eos &E, double e) {
     if (E.kind == 0) return e * kThird;
+    if (E.kind == 2) return wb_pressure(e);
     double p;
     table_lookup(E, e, &p, nullptr);
     return p;
@@ -132,6 +135,7 @@ __device__ __forceinline__ double eos_p(const Eos &E, double e) {
 
 __device__ __forceinline__ double eos_T(const Eos &E, double e) {
     if (E.kind == 0) return conformal_T(e, E.powf_fix);
+    if (E.kind == 2) return wb_temperature(e);
     double T;
     table_lookup(E, e, nullptr, &T);
     return T;
@@ -185,7 +189,7 @@ __device__ __forceinline__ double rbar0(const Powers &w) {
     constexpr double d[9] = {0.05470696094814806, 14.505878005231883, 522.6643024173569, 2731.7776413939037,
                              -6161.1991042880445, -3989.4375208972588, 15008.260526258282, -10243.036679405379,
                              2116.74060159494};
-    return psum(n, w) / psum(d, w);
+    return fdiv(psum(n, w), psum(d, w));
 }
 
 __device__ __forceinline__ Fits fits(const Powers &w) {
@@ -212,16 +216,16 @@ __device__ __forceinline__ Fits fits(const Powers &w) {
                                  29824.022790048104,     -157419.8447785501,   641300.6529027821,
                                  -2.0248032895288002e6};
     Fits f;
-    f.Rtilde = psum(rt_n, w) / psum(rt_d, w);
+    f.Rtilde = fdiv(psum(rt_n, w), psum(rt_d, w));
     f.Rbar0 = rbar0(w);
-    f.Rgamma = psum(rg_n, w) / psum(rg_d, w);
-    f.Rbar0P = psum(rp_n, rp_k, w) / psum(rp_d, rp_k, w);
+    f.Rgamma = fdiv(psum(rg_n, w), psum(rg_d, w));
+    f.Rbar0P = fdiv(psum(rp_n, rp_k, w), psum(rp_d, rp_k, w));
     return f;
 }
 
 // transversePressureHat (AnisotropicDistributionFunctions.cpp:5-24)
 __device__ __forceinline__ double pt_hat(double e, double p, double pl) {
-    return 0.5 * ((e - pl) - rbar0(powers_of(pl / e)) * (e - 3.0 * p));
+    return 0.5 * ((e - pl) - rbar0(powers_of(fdiv(pl, e))) * (e - 3.0 * p));
 }
 
 // ---------------------------------------------------------------------------
@@ -229,19 +233,21 @@ __device__ __forceinline__ double pt_hat(double e, double p, double pl) {
 // two values is sign*min(|x|,|y|) when the signs (with sign(0) = +1) agree,
 // else 0 -- the reference's (sign(x)+sign(y))*min/2 without the arithmetic.
 // ---------------------------------------------------------------------------
-template <typename R>
-__device__ __forceinline__ R minmod2(R x, R y) {
-    R m = fmin(fabs(x), fabs(y));
-    bool px = !(x < R(0)), py = !(y < R(0));
-    return px == py ? (px ? m : -m) : R(0);
-}
+__device__ __forceinline__ bool same_sign(double a, double b) { return (__double2hiint(a) ^ __double2hiint(b)) >= 0; }
+__device__ __forceinline__ bool same_sign(float a, float b) { return (__float_as_int(a) ^ __float_as_int(b)) >= 0; }
 
+// minmod(l, minmod(c, r)) of l = theta (y - x), c = (z - x) / 2,
+// r = theta (z - y).  It is nonzero only when l and r are nonzero with one
+// sign (c then has that sign as well); it is then the smallest of the three
+// magnitudes with that sign.  In every other case -- opposite signs, or a
+// zero, whose sign the reference counts as +1 -- one of the minima is 0.
 template <typename R>
 __device__ __forceinline__ R slope(R x, R y, R z) {
-    R l = R(1.1) * (y - x);
-    R c = R(0.5) * (z - x);
-    R r = R(1.1) * (z - y);
-    return minmod2(l, minmod2(c, r));
+    const R l = R(1.1) * (y - x);
+    const R c = R(0.5) * (z - x);
+    const R r = R(1.1) * (z - y);
+    const R m = fmin(fmin(fabs(l), fabs(r)), fabs(c));
+    return (m > R(0) && same_sign(l, r)) ? __builtin_copysign(m, l) : R(0);
 }
 
 // ---------------------------------------------------------------------------

@@ -41,6 +41,7 @@ struct HostEos {
 
 inline double host_eos_p(const HostEos &E, double e) {
     if (E.kind == VAH_EOS_CONFORMAL) return e / 3;
+    if (E.kind == VAH_EOS_WB) return vah::wb_pressure(e);
     double f = (log(e) - E.lne0) * (1.0 / E.h);
     int last = E.n - 1;
     if (!(f >= 0.0)) return E.p[0] * (e / exp(E.lne0));

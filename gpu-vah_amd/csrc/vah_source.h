@@ -281,7 +281,7 @@ __device__ __forceinline__ void regulate(double t, QA &q, double &e, double &p, 
     const double ptHat = pt_hat(e, p, pl);
     const double pitt = q[PITT], pitx = q[PITX], pity = q[PITY], pitn = q[PITN], pixx = q[PIXX], pixy = q[PIXY],
                  pixn = q[PIXN], piyy = q[PIYY], piyn = q[PIYN], pinn = q[PINN];
-    const double isF = 1.0 / __builtin_sqrt(1.0 + ux * ux + uy * uy);
+    const double isF = rsqrt(1.0 + ux * ux + uy * uy);
     const double z0 = t * un * isF;
     const double z3 = ut / t * isF;
     const double t2 = t * t;
@@ -299,11 +299,13 @@ __device__ __forceinline__ void regulate(double t, QA &q, double &e, double &p, 
                           z0 * pitx - t2 * z3 * pixn,
                           z0 * pity - t2 * z3 * piyn,
                           z0 * pitn - t2 * z3 * pinn};
-    // ratios to spipi; a zero spipi makes them 0/0 = NaN, which never wins
+    // ratios to spipi; a zero spipi makes them 0 * inf = NaN (or +-inf for a
+    // nonzero numerator), as the quotients would be
     double rho = spipi / sTrsTrs;
+    const double ispipi = 1.0 / spipi;
 #pragma unroll
     for (int m = 0; m < 9; ++m) {
-        double am = ar[m] / spipi;
+        double am = ar[m] * ispipi;
         if (am > rho) rho = am;
     }
     double fac = tanh(rho) / rho;

@@ -157,20 +157,75 @@ __device__ __forceinline__ void face_speeds(double t, double guess, const FaceSt
     vB = speed(B, ub, sb);
 }
 
-// Flux through the face between cells c-st and c.  The left/right states are
-// the reference's half-cell extrapolations (HalfSiteExtrapolation.cpp:12-26);
-// emit(n, H_n) receives the 15 components in order.  The nine components the
-// Newton solve needs stay in registers, the other six are reconstructed after
-// it, which keeps the live state small.
+// ---------------------------------------------------------------------------
+// Field sets seen through buffer resources.  A load or store takes the byte
+// offset of the cell (per lane, 32-bit) and the byte offset of the field
+// (uniform, an SGPR), so the 15-field stencils need no per-lane 64-bit address
+// arithmetic.  An access outside the set's byte range reads 0 and drops the
+// store instead of faulting.  A set must stay below 4 GB (checked on the host).
+// ---------------------------------------------------------------------------
+typedef unsigned int u32x2_t __attribute__((ext_vector_type(2)));
+
+constexpr int kBufWord3 = 0x00020000;  // raw dword access, gfx9 family
+
+struct Buf {
+    __amdgpu_buffer_rsrc_t r;
+};
+
+__device__ __forceinline__ Buf make_buf(const double *base, int nfields, long fs) {
+    Buf b;
+    b.r = __builtin_amdgcn_make_buffer_rsrc((void *)base, 0, (int)(uint32_t)((long)nfields * fs * 8), kBufWord3);
+    return b;
+}
+
+__device__ __forceinline__ double ld(const Buf &b, uint32_t cell, uint32_t field) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(b.r, (int)cell, (int)field, 0));
+}
+
+__device__ __forceinline__ void st(const Buf &b, uint32_t cell, uint32_t field, double v) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2_t, v), b.r, (int)cell, (int)field, 0);
+}
+
+// the buffers of one stage, and the byte strides of the layout
+struct Views {
+    Buf qin, qout, qn, e_in, p_in, e_out, p_out, ucur, uprev, uguess, uout;
+    uint32_t f8;        // field to field
+    uint32_t sy8, sz8;  // row to row, plane to plane
+};
+
+__device__ __forceinline__ Views make_views(const StageParams &P) {
+    Views v;
+    v.qin = make_buf(P.qin, NQ, P.fs);
+    v.qout = make_buf(P.qout, NQ, P.fs);
+    v.qn = make_buf(P.qn, NQ, P.fs);
+    v.e_in = make_buf(P.e_in, 1, P.fs);
+    v.p_in = make_buf(P.p_in, 1, P.fs);
+    v.e_out = make_buf(P.e_out, 1, P.fs);
+    v.p_out = make_buf(P.p_out, 1, P.fs);
+    v.ucur = make_buf(P.ucur, 4, P.fs);
+    v.uprev = make_buf(P.uprev, 4, P.fs);
+    v.uguess = make_buf(P.uguess, 4, P.fs);
+    v.uout = make_buf(P.uout, 4, P.fs);
+    v.f8 = (uint32_t)(P.fs * 8);
+    v.sy8 = (uint32_t)(P.sy * 8);
+    v.sz8 = (uint32_t)(P.sz * 8);
+    return v;
+}
+
+// Flux through the face between cells c-st and c (byte offsets).  The
+// left/right states are the reference's half-cell extrapolations
+// (HalfSiteExtrapolation.cpp:12-26); emit(n, H_n) receives the 15 components
+// in order.  The nine components the Newton solve needs stay in registers,
+// the other six are reconstructed after it, which keeps the live state small.
 template <int DIR, typename R, typename Emit>
-__device__ __forceinline__ void face_flux(double t, double uT_guess, const double *__restrict__ q, long fs, long c,
-                                          long st, Emit emit) {
+__device__ __forceinline__ void face_flux(double t, double uT_guess, const Buf &q, uint32_t f8, uint32_t c,
+                                          uint32_t stp, Emit emit) {
     constexpr int NN = 9;  // TTT..PITN
     R qL[NN], qR[NN];
 #pragma unroll
     for (int n = 0; n < NN; ++n) {
-        const double *a = q + n * fs;
-        R q0 = (R)a[c - 2 * st], q1 = (R)a[c - st], q2 = (R)a[c], q3 = (R)a[c + st];
+        const uint32_t fo = n * f8;
+        R q0 = (R)ld(q, c - 2 * stp, fo), q1 = (R)ld(q, c - stp, fo), q2 = (R)ld(q, c, fo), q3 = (R)ld(q, c + stp, fo);
         qL[n] = q1 + R(0.5) * slope(q0, q1, q2);
         qR[n] = q2 - R(0.5) * slope(q1, q2, q3);
     }
@@ -186,23 +241,23 @@ __device__ __forceinline__ void face_flux(double t, double uT_guess, const doubl
     for (int n = 0; n < NN; ++n) emit(n, cR * qR[n] + cL * qL[n]);
 #pragma unroll
     for (int n = NN; n < NQ; ++n) {
-        const double *a = q + n * fs;
-        R q0 = (R)a[c - 2 * st], q1 = (R)a[c - st], q2 = (R)a[c], q3 = (R)a[c + st];
+        const uint32_t fo = n * f8;
+        R q0 = (R)ld(q, c - 2 * stp, fo), q1 = (R)ld(q, c - stp, fo), q2 = (R)ld(q, c, fo), q3 = (R)ld(q, c + stp, fo);
         R l = q1 + R(0.5) * slope(q0, q1, q2);
         R r = q2 - R(0.5) * slope(q1, q2, q3);
         emit(n, cR * r + cL * l);
     }
 }
 
-__device__ __forceinline__ double uT_of(const double *__restrict__ u, long fs, long s) {
-    double ux = u[fs + s], uy = u[2 * fs + s];
+__device__ __forceinline__ double uT_of(const Buf &u, uint32_t f8, uint32_t c) {
+    double ux = ld(u, c, f8), uy = ld(u, c, 2 * f8);
     return __builtin_sqrt(fabs(ux * ux + uy * uy));
 }
 
 // d_i of the pi components entering the axis source of direction DIR
 template <int DIR, typename R>
-__device__ __forceinline__ AxisPi<R> axis_pi(const double *__restrict__ q, long fs, long s, long st, double h) {
-    auto d = [&](int n) { return (R)((q[n * fs + s + st] - q[n * fs + s - st]) * h); };
+__device__ __forceinline__ AxisPi<R> axis_pi(const Buf &q, uint32_t f8, uint32_t s, uint32_t stp, double h) {
+    auto d = [&](int n) { return (R)((ld(q, s + stp, n * f8) - ld(q, s - stp, n * f8)) * h); };
     AxisPi<R> r;
     r.dtt = d(PITT);
     r.dtx = d(PITX);
@@ -235,46 +290,46 @@ __device__ __forceinline__ void add_component(double &acc, int n, R Hm, R Hp, co
 }
 
 template <bool D3>
-__device__ __forceinline__ void load_src_in(const StageParams &P, long s, SrcIn &in) {
-    const double *u = P.ucur;
-    const long fs = P.fs, sy = P.sy, sz = P.sz;
+__device__ __forceinline__ void load_src_in(const StageParams &P, const Views &V, uint32_t s, SrcIn &in) {
+    const Buf &u = V.ucur;
+    const uint32_t f8 = V.f8, sy8 = V.sy8, sz8 = V.sz8;
     in.t = P.t_src;
     in.dt_inv = P.dt_inv;
     in.etabar_inv = P.etabar_inv;
-    in.e = P.e_in[s];
-    in.p = P.p_in[s];
-    in.ut = u[s];
-    in.ux = u[fs + s];
-    in.uy = u[2 * fs + s];
-    in.un = u[3 * fs + s];
-    in.utp = P.uprev[s];
-    in.uxp = P.uprev[fs + s];
-    in.uyp = P.uprev[2 * fs + s];
-    in.unp = P.uprev[3 * fs + s];
-    auto cd = [&](const double *f, long st, double h) { return (f[s + st] - f[s - st]) * h; };
-    in.dxut = cd(u, 1, P.hx);
-    in.dxux = cd(u + fs, 1, P.hx);
-    in.dxuy = cd(u + 2 * fs, 1, P.hx);
-    in.dxun = cd(u + 3 * fs, 1, P.hx);
-    in.dyut = cd(u, sy, P.hy);
-    in.dyux = cd(u + fs, sy, P.hy);
-    in.dyuy = cd(u + 2 * fs, sy, P.hy);
-    in.dyun = cd(u + 3 * fs, sy, P.hy);
-    const double *pl = P.qin + PL * fs;
-    in.dxp = cd(P.p_in, 1, P.hx);
-    in.dyp = cd(P.p_in, sy, P.hy);
-    in.dxe = cd(P.e_in, 1, P.hx);
-    in.dye = cd(P.e_in, sy, P.hy);
-    in.dxpl = cd(pl, 1, P.hx);
-    in.dypl = cd(pl, sy, P.hy);
+    in.e = ld(V.e_in, s, 0);
+    in.p = ld(V.p_in, s, 0);
+    in.ut = ld(u, s, 0);
+    in.ux = ld(u, s, f8);
+    in.uy = ld(u, s, 2 * f8);
+    in.un = ld(u, s, 3 * f8);
+    in.utp = ld(V.uprev, s, 0);
+    in.uxp = ld(V.uprev, s, f8);
+    in.uyp = ld(V.uprev, s, 2 * f8);
+    in.unp = ld(V.uprev, s, 3 * f8);
+    auto cd = [&](const Buf &b, uint32_t fo, uint32_t stp, double h) { return (ld(b, s + stp, fo) - ld(b, s - stp, fo)) * h; };
+    in.dxut = cd(u, 0, 8, P.hx);
+    in.dxux = cd(u, f8, 8, P.hx);
+    in.dxuy = cd(u, 2 * f8, 8, P.hx);
+    in.dxun = cd(u, 3 * f8, 8, P.hx);
+    in.dyut = cd(u, 0, sy8, P.hy);
+    in.dyux = cd(u, f8, sy8, P.hy);
+    in.dyuy = cd(u, 2 * f8, sy8, P.hy);
+    in.dyun = cd(u, 3 * f8, sy8, P.hy);
+    const uint32_t plo = PL * f8;
+    in.dxp = cd(V.p_in, 0, 8, P.hx);
+    in.dyp = cd(V.p_in, 0, sy8, P.hy);
+    in.dxe = cd(V.e_in, 0, 8, P.hx);
+    in.dye = cd(V.e_in, 0, sy8, P.hy);
+    in.dxpl = cd(V.qin, plo, 8, P.hx);
+    in.dypl = cd(V.qin, plo, sy8, P.hy);
     if (D3) {
-        in.dnut = cd(u, sz, P.hz);
-        in.dnux = cd(u + fs, sz, P.hz);
-        in.dnuy = cd(u + 2 * fs, sz, P.hz);
-        in.dnun = cd(u + 3 * fs, sz, P.hz);
-        in.dnp = cd(P.p_in, sz, P.hz);
-        in.dne = cd(P.e_in, sz, P.hz);
-        in.dnpl = cd(pl, sz, P.hz);
+        in.dnut = cd(u, 0, sz8, P.hz);
+        in.dnux = cd(u, f8, sz8, P.hz);
+        in.dnuy = cd(u, 2 * f8, sz8, P.hz);
+        in.dnun = cd(u, 3 * f8, sz8, P.hz);
+        in.dnp = cd(V.p_in, 0, sz8, P.hz);
+        in.dne = cd(V.e_in, 0, sz8, P.hz);
+        in.dnpl = cd(V.qin, plo, sz8, P.hz);
     } else {
         // one eta cell: the ghost planes equal the cell, the differences are exactly 0
         in.dnut = in.dnux = in.dnuy = in.dnun = 0.0;
@@ -284,14 +339,14 @@ __device__ __forceinline__ void load_src_in(const StageParams &P, long s, SrcIn 
 
 // Stage-2 convex combination, recovery, regulation, store with ghost copies.
 template <bool D3, bool STAGE2>
-__device__ __forceinline__ double finish_cell(const StageParams &P, long s, int ii, int jj, int kk, double (&acc)[NQ],
-                                            bool &nan, bool &fallback) {
-    const long fs = P.fs;
+__device__ __forceinline__ double finish_cell(const StageParams &P, const Views &V, uint32_t s, int ii, int jj, int kk,
+                                              double (&acc)[NQ], bool &nan, bool &fallback) {
+    const uint32_t f8 = V.f8;
     if (STAGE2) {
 #pragma unroll
-        for (int n = 0; n < NQ; ++n) acc[n] = 0.5 * (acc[n] + P.qn[n * fs + s]);
+        for (int n = 0; n < NQ; ++n) acc[n] = 0.5 * (acc[n] + ld(V.qn, s, n * f8));
     }
-    Prim r = recover(P.eos, P.t_rec, acc, P.e_in[s], uT_of(P.uguess, fs, s));
+    Prim r = recover(P.eos, P.t_rec, acc, ld(V.e_in, s, 0), uT_of(V.uguess, f8, s));
     nan = nan || prim_nan(r);
     fallback = !r.ok;
     double e = r.e, p = r.p;
@@ -305,23 +360,23 @@ __device__ __forceinline__ double finish_cell(const StageParams &P, long s, int 
     vals[NQ + 3] = r.ux;
     vals[NQ + 4] = r.uy;
     vals[NQ + 5] = r.un;
-    auto store = [&](long d) {
+    auto store = [&](uint32_t d) {
 #pragma unroll
-        for (int n = 0; n < NQ; ++n) P.qout[n * fs + d] = vals[n];
-        P.e_out[d] = vals[NQ];
-        P.p_out[d] = vals[NQ + 1];
+        for (int n = 0; n < NQ; ++n) st(V.qout, d, n * f8, vals[n]);
+        st(V.e_out, d, 0, vals[NQ]);
+        st(V.p_out, d, 0, vals[NQ + 1]);
 #pragma unroll
-        for (int m = 0; m < 4; ++m) P.uout[m * fs + d] = vals[NQ + 2 + m];
+        for (int m = 0; m < 4; ++m) st(V.uout, d, m * f8, vals[NQ + 2 + m]);
     };
     store(s);
     // face ghosts: copies of the boundary cell (only one index outside)
-    if (ii == 0) { store(s - 1); store(s - 2); }
-    if (ii == P.nx - 1) { store(s + 1); store(s + 2); }
-    if (jj == 0) { store(s - P.sy); store(s - 2 * P.sy); }
-    if (jj == P.ny - 1) { store(s + P.sy); store(s + 2 * P.sy); }
+    if (ii == 0) { store(s - 8); store(s - 16); }
+    if (ii == P.nx - 1) { store(s + 8); store(s + 16); }
+    if (jj == 0) { store(s - V.sy8); store(s - 2 * V.sy8); }
+    if (jj == P.ny - 1) { store(s + V.sy8); store(s + 2 * V.sy8); }
     if (D3) {
-        if (kk == 0) { store(s - P.sz); store(s - 2 * P.sz); }
-        if (kk == P.nz - 1) { store(s + P.sz); store(s + 2 * P.sz); }
+        if (kk == 0) { store(s - V.sz8); store(s - 2 * V.sz8); }
+        if (kk == P.nz - 1) { store(s + V.sz8); store(s + 2 * V.sz8); }
     }
     return e;
 }
@@ -361,8 +416,10 @@ __device__ __forceinline__ void stage_tile(const StageParams &P, int tile, R (*H
     const int k0 = zc * P.kz;
     const int k1 = min(k0 + P.kz, P.nz);
 
+    const Views V = make_views(P);
+    const uint32_t f8 = V.f8, sy8 = V.sy8, sz8 = V.sz8;
     const int tid = (int)threadIdx.x;
-    const long fs = P.fs, sy = P.sy, sz = P.sz;
+    const uint32_t sy = (uint32_t)P.sy;
     const double t = P.t_src, dt = P.dt;
     const R inv_dx = (R)P.inv_dx, inv_dy = (R)P.inv_dy, inv_dz = (R)P.inv_dz;
 
@@ -372,8 +429,8 @@ __device__ __forceinline__ void stage_tile(const StageParams &P, int tile, R (*H
     const bool cell_ok = cj < TY && ii < P.nx && jj < P.ny;
 
     for (int kk = k0; kk < k1; ++kk) {
-        const long plane = (long)plane_of(kk, D3) * sz;
-        const long s = plane + (long)(jj + 2) * sy + (ii + 2);
+        const uint32_t plane = (uint32_t)plane_of(kk, D3) * (uint32_t)P.sz;
+        const uint32_t s = (plane + (uint32_t)(jj + 2) * sy + (uint32_t)(ii + 2)) * 8u;
         bool nan = false, fallback = false;
         double acc[NQ];
 
@@ -381,24 +438,24 @@ __device__ __forceinline__ void stage_tile(const StageParams &P, int tile, R (*H
         if (cell_ok) {
             double Q[NQ];
 #pragma unroll
-            for (int n = 0; n < NQ; ++n) Q[n] = P.qin[n * fs + s];
+            for (int n = 0; n < NQ; ++n) Q[n] = ld(V.qin, s, n * f8);
             SrcIn in;
-            load_src_in<D3>(P, s, in);
+            load_src_in<D3>(P, V, s, in);
             R S[NQ];
             source_terms<R>(P.eos, in, Q, S, nan);
 #pragma unroll
             for (int n = 0; n < NQ; ++n) acc[n] = Q[n] + dt * (double)S[n];
         }
-        const double iut = cell_ok ? rcp(P.ucur[s]) : 0.0;
+        const double iut = cell_ok ? rcp(ld(V.ucur, s, 0)) : 0.0;
 
         if (SHARED) {
-            // ---- x faces: 17 x 15, face f between cells (fi-1, fj) and (fi, fj)
+            // ---- x faces: (TX+1) x TY, face f between cells (fi-1, fj) and (fi, fj)
             {
                 const int fi = tid % (TX + 1), fj = tid / (TX + 1);
                 const int gi = i0 + fi, gj = j0 + fj;  // interior index of the right cell
                 if (fj < TY && gi <= P.nx && gj < P.ny) {
-                    const long c = plane + (long)(gj + 2) * sy + (gi + 2);
-                    face_flux<1, R>(t, uT_of(P.ucur, fs, c - 1), P.qin, fs, c, 1,
+                    const uint32_t c = (plane + (uint32_t)(gj + 2) * sy + (uint32_t)(gi + 2)) * 8u;
+                    face_flux<1, R>(t, uT_of(V.ucur, f8, c - 8), V.qin, f8, c, 8,
                                     [&](int n, R h) { Hs[n][tid] = h; });
                 }
             }
@@ -406,18 +463,18 @@ __device__ __forceinline__ void stage_tile(const StageParams &P, int tile, R (*H
             if (cell_ok) {
                 const int fm = cj * (TX + 1) + ci;
                 R Sa[4];
-                axis_source<1>(axis_pi<1, R>(P.qin, fs, s, 1, P.hx), (R)(P.ucur[fs + s] * iut), Sa);
+                axis_source<1>(axis_pi<1, R>(V.qin, f8, s, 8, P.hx), (R)(ld(V.ucur, s, f8) * iut), Sa);
 #pragma unroll
                 for (int n = 0; n < NQ; ++n) add_component(acc[n], n, Hs[n][fm], Hs[n][fm + 1], Sa, inv_dx, dt);
             }
             __syncthreads();
-            // ---- y faces: 16 x 16, face f between cells (fi, fj-1) and (fi, fj)
+            // ---- y faces: TX x (TY+1), face f between cells (fi, fj-1) and (fi, fj)
             {
                 const int fi = tid % TX, fj = tid / TX;
                 const int gi = i0 + fi, gj = j0 + fj;
                 if (fj <= TY && gi < P.nx && gj <= P.ny) {
-                    const long c = plane + (long)(gj + 2) * sy + (gi + 2);
-                    face_flux<2, R>(t, uT_of(P.ucur, fs, c - sy), P.qin, fs, c, sy,
+                    const uint32_t c = (plane + (uint32_t)(gj + 2) * sy + (uint32_t)(gi + 2)) * 8u;
+                    face_flux<2, R>(t, uT_of(V.ucur, f8, c - sy8), V.qin, f8, c, sy8,
                                     [&](int n, R h) { Hs[n][tid] = h; });
                 }
             }
@@ -425,7 +482,7 @@ __device__ __forceinline__ void stage_tile(const StageParams &P, int tile, R (*H
             if (cell_ok) {
                 const int fm = cj * TX + ci;
                 R Sa[4];
-                axis_source<2>(axis_pi<2, R>(P.qin, fs, s, sy, P.hy), (R)(P.ucur[2 * fs + s] * iut), Sa);
+                axis_source<2>(axis_pi<2, R>(V.qin, f8, s, sy8, P.hy), (R)(ld(V.ucur, s, 2 * f8) * iut), Sa);
 #pragma unroll
                 for (int n = 0; n < NQ; ++n) add_component(acc[n], n, Hs[n][fm], Hs[n][fm + TX], Sa, inv_dy, dt);
             }
@@ -433,43 +490,43 @@ __device__ __forceinline__ void stage_tile(const StageParams &P, int tile, R (*H
             // ---- eta faces: solved by the cell's own lane, the upper one carried to the next plane
             if (D3 && cell_ok) {
                 if (kk == k0)
-                    face_flux<3, R>(t, uT_of(P.ucur, fs, s - sz), P.qin, fs, s, sz,
+                    face_flux<3, R>(t, uT_of(V.ucur, f8, s - sz8), V.qin, f8, s, sz8,
                                     [&](int n, R h) { Hz[n][tid] = h; });
                 R Sa[4];
-                axis_source<3>(axis_pi<3, R>(P.qin, fs, s, sz, P.hz), (R)(P.ucur[3 * fs + s] * iut), Sa);
-                face_flux<3, R>(t, uT_of(P.ucur, fs, s), P.qin, fs, s + sz, sz, [&](int n, R h) {
+                axis_source<3>(axis_pi<3, R>(V.qin, f8, s, sz8, P.hz), (R)(ld(V.ucur, s, 3 * f8) * iut), Sa);
+                face_flux<3, R>(t, uT_of(V.ucur, f8, s), V.qin, f8, s + sz8, sz8, [&](int n, R h) {
                     add_component(acc[n], n, Hz[n][tid], h, Sa, inv_dz, dt);
                     Hz[n][tid] = h;
                 });
             }
         } else if (cell_ok) {
             // reference work split: both faces of each direction, Newton guess from this cell
-            const double uTs = uT_of(P.ucur, fs, s);
+            const double uTs = uT_of(V.ucur, f8, s);
             {
                 R Sa[4];
-                axis_source<1>(axis_pi<1, R>(P.qin, fs, s, 1, P.hx), (R)(P.ucur[fs + s] * iut), Sa);
-                face_flux<1, R>(t, uTs, P.qin, fs, s + 1, 1, [&](int n, R h) { Hs[n][tid] = h; });
-                face_flux<1, R>(t, uTs, P.qin, fs, s, 1,
+                axis_source<1>(axis_pi<1, R>(V.qin, f8, s, 8, P.hx), (R)(ld(V.ucur, s, f8) * iut), Sa);
+                face_flux<1, R>(t, uTs, V.qin, f8, s + 8, 8, [&](int n, R h) { Hs[n][tid] = h; });
+                face_flux<1, R>(t, uTs, V.qin, f8, s, 8,
                                 [&](int n, R h) { add_component(acc[n], n, h, Hs[n][tid], Sa, inv_dx, dt); });
             }
             {
                 R Sa[4];
-                axis_source<2>(axis_pi<2, R>(P.qin, fs, s, sy, P.hy), (R)(P.ucur[2 * fs + s] * iut), Sa);
-                face_flux<2, R>(t, uTs, P.qin, fs, s + sy, sy, [&](int n, R h) { Hs[n][tid] = h; });
-                face_flux<2, R>(t, uTs, P.qin, fs, s, sy,
+                axis_source<2>(axis_pi<2, R>(V.qin, f8, s, sy8, P.hy), (R)(ld(V.ucur, s, 2 * f8) * iut), Sa);
+                face_flux<2, R>(t, uTs, V.qin, f8, s + sy8, sy8, [&](int n, R h) { Hs[n][tid] = h; });
+                face_flux<2, R>(t, uTs, V.qin, f8, s, sy8,
                                 [&](int n, R h) { add_component(acc[n], n, h, Hs[n][tid], Sa, inv_dy, dt); });
             }
             if (D3) {
                 R Sa[4];
-                axis_source<3>(axis_pi<3, R>(P.qin, fs, s, sz, P.hz), (R)(P.ucur[3 * fs + s] * iut), Sa);
-                face_flux<3, R>(t, uTs, P.qin, fs, s + sz, sz, [&](int n, R h) { Hs[n][tid] = h; });
-                face_flux<3, R>(t, uTs, P.qin, fs, s, sz,
+                axis_source<3>(axis_pi<3, R>(V.qin, f8, s, sz8, P.hz), (R)(ld(V.ucur, s, 3 * f8) * iut), Sa);
+                face_flux<3, R>(t, uTs, V.qin, f8, s + sz8, sz8, [&](int n, R h) { Hs[n][tid] = h; });
+                face_flux<3, R>(t, uTs, V.qin, f8, s, sz8,
                                 [&](int n, R h) { add_component(acc[n], n, h, Hs[n][tid], Sa, inv_dz, dt); });
             }
         }
 
         double e_new = 0.0;
-        if (cell_ok) e_new = finish_cell<D3, STAGE2>(P, s, ii, jj, kk, acc, nan, fallback);
+        if (cell_ok) e_new = finish_cell<D3, STAGE2>(P, V, s, ii, jj, kk, acc, nan, fallback);
         count_flags(P, nan, fallback, e_new);
     }
 }

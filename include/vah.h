@@ -57,7 +57,9 @@ typedef struct vah_hydro {
 
 enum vah_eos_kind {
     VAH_EOS_CONFORMAL = 0,     /* p = e/3, T = powf(e/13.8997, 1/4) as the reference */
-    VAH_EOS_TABLE = 1          /* tabulated e -> (p, T), linear in ln e */
+    VAH_EOS_TABLE = 1,         /* tabulated e -> (p, T), linear in ln e */
+    VAH_EOS_WB = 2             /* the reference's Wuppertal-Budapest fits p(e), T(e)
+                                  (old_cpu-vah_code/src/EquationOfState.cpp:31-76,120-142) */
 };
 
 enum vah_precision {

@@ -65,6 +65,11 @@ def lib():
         L.orc_nan_hits.argtypes = [ctypes.c_void_p]
         L.orc_nan_hits.restype = ctypes.c_long
         L.orc_get_state.argtypes = [ctypes.c_void_p] + [_dp] * 5
+        L.orc_set_eos_wb.argtypes = [ctypes.c_void_p]
+        L.orc_set_eos_wb.restype = None
+        for name in ("orc_wb_pressure", "orc_wb_temperature"):
+            getattr(L, name).argtypes = [ctypes.c_double]
+            getattr(L, name).restype = ctypes.c_double
         for name in ("orc_temperature_conformal", "orc_rtilde", "orc_rbar0", "orc_rgamma", "orc_rbar0p"):
             getattr(L, name).argtypes = [ctypes.c_double]
             getattr(L, name).restype = ctypes.c_double
@@ -113,7 +118,7 @@ class Oracle:
     (index = i + nx*(j + ny*k)), exactly as the product's C-ABI takes them.
     """
 
-    def __init__(self, nx, ny, nz, dx, dy, dz, dt, etabar, t0, threads=None, eos_table=None):
+    def __init__(self, nx, ny, nz, dx, dy, dz, dt, etabar, t0, threads=None, eos_table=None, eos_wb=False):
         L = lib()
         self.nx, self.ny, self.nz = nx, ny, nz
         self.n = nx * ny * nz
@@ -128,6 +133,8 @@ class Oracle:
             T = np.ascontiguousarray(T, dtype=np.float64)
             if L.orc_set_eos_table(self.h, len(p), lne0, h, _ptr(p), _ptr(T)) != 0:
                 raise MemoryError("oracle EOS table allocation failed")
+        if eos_wb:
+            L.orc_set_eos_wb(self.h)
 
     def close(self):
         if self.h:

@@ -41,6 +41,9 @@
  *
  * The tabulated equation of state (EOS_TABLE) is not in the reference: it is
  * this project's lattice-QCD-like EOS, looked up exactly as the product does.
+ * EOS_WB is the reference's non-conformal branch (the Wuppertal-Budapest
+ * rational fits, EquationOfState.cpp:31-76 and :120-142), active there when
+ * CONFORMAL_EOS is not defined.
  */
 #include <math.h>
 #include <stdint.h>
@@ -57,6 +60,7 @@ enum { TTT, TTX, TTY, TTN, PL, PITT, PITX, PITY, PITN, PIXX, PIXY, PIXN, PIYY, P
 
 #define EOS_CONFORMAL 0
 #define EOS_TABLE 1
+#define EOS_WB 2
 
 typedef struct {
     int kind;
@@ -114,9 +118,61 @@ static inline void eos_table_lookup(const orc_eos *E, double e, double *p, doubl
     if (T) *T = E->T[i] + w * (E->T[i + 1] - E->T[i]);
 }
 
+/* Wuppertal-Budapest p(e) (EquationOfState.cpp:31-76): a degree-12 rational
+ * function in e, numerator and denominator each a nested fma chain (Horner
+ * in the explicit powers), quotient in double. */
+static const double WB_PA[13] = {-0.25181736420168666, 9737.845799644809, 1.077580993288114e6,
+                                 3.1729694865420084e6, 1.6357487344679043e6, 334334.4309240126,
+                                 41913.439282708554, 6340.448389300905, 141.5073484468774,
+                                 0.7158279081255019, 0.0009417586777847889, 3.1188455176941583e-7,
+                                 1.9531729608963267e-11};
+static const double WB_PB[13] = {45829.44617893836, 4.0574329080826794e6, 2.0931169138134286e7,
+                                 1.3512402226067686e7, 1.7851642641834426e6, 278581.2989342773,
+                                 26452.34905933697, 499.04919730607065, 2.3405487982094204,
+                                 0.002962497695527404, 9.601103399348206e-7, 5.928138360995685e-11,
+                                 3.2581066229887368e-18};
+/* effectiveTemperature (EquationOfState.cpp:120-142): degree 11 / 11, the
+ * terms added left to right */
+static const double WB_TN[12] = {1.510073201405604e-29, 8.014062800678687e-18, 2.4954778310451065e-10,
+                                 0.000063810382643387, 0.4873490574161924, 207.48582344326206,
+                                 6686.07424325115, 14109.766109389702, 1471.6180520527757,
+                                 14.055788949565482, 0.015421252394182246, 1.5780479034557783e-6};
+static const double WB_TD[12] = {7.558667139355393e-28, 1.3686372302041508e-16, 2.998130743142826e-9,
+                                 0.0005036835870305458, 2.316902328874072, 578.0778724946719,
+                                 11179.193315394154, 17965.67607192861, 1051.0730543534657,
+                                 5.916312075925817, 0.003778342768228011, 1.8472801679382593e-7};
+
+static double wb_pressure(double e) {
+    double pw[13];
+    pw[0] = 1.0;
+    pw[1] = e;
+    for (int k = 2; k <= 12; ++k) pw[k] = pw[k - 1] * e;
+    double a = WB_PA[0], b = WB_PB[0];
+    /* innermost term first: fma(a1, e, a0), then fma(a2, e^2, .) ... */
+    for (int k = 1; k <= 12; ++k) {
+        a = fma(WB_PA[k], pw[k], a);
+        b = fma(WB_PB[k], pw[k], b);
+    }
+    return a / b;
+}
+
+static double wb_temperature(double e) {
+    double pw[12];
+    pw[0] = 1.0;
+    pw[1] = e;
+    for (int k = 2; k <= 11; ++k) pw[k] = pw[k - 1] * e;
+    double n = WB_TN[0], d = WB_TD[0];
+    for (int k = 1; k <= 11; ++k) {
+        n = n + WB_TN[k] * pw[k];
+        d = d + WB_TD[k] * pw[k];
+    }
+    return n / d;
+}
+
 /* EquationOfState.cpp:78 */
 static inline double eos_pressure(const orc_eos *E, double e) {
     if (E->kind == EOS_CONFORMAL) return e / 3;
+    if (E->kind == EOS_WB) return wb_pressure(e);
     double p;
     eos_table_lookup(E, e, &p, NULL);
     return p;
@@ -125,6 +181,7 @@ static inline double eos_pressure(const orc_eos *E, double e) {
 /* EquationOfState.cpp:144: powf of the float-rounded argument, widened. */
 static inline double eos_temperature(const orc_eos *E, double e) {
     if (E->kind == EOS_CONFORMAL) return (double)powf((float)(e / EOS_FACTOR), 0.25f);
+    if (E->kind == EOS_WB) return wb_temperature(e);
     double T;
     eos_table_lookup(E, e, NULL, &T);
     return T;
@@ -877,6 +934,17 @@ void orc_destroy(orc_ctx *c) {
 }
 
 void orc_set_threads(orc_ctx *c, int n) { c->nthreads = n > 0 ? n : 1; }
+
+/* The reference's Wuppertal-Budapest fits in place of the conformal EOS. */
+void orc_set_eos_wb(orc_ctx *c) {
+    free(c->eos_store);
+    c->eos_store = NULL;
+    memset(&c->eos, 0, sizeof c->eos);
+    c->eos.kind = EOS_WB;
+}
+
+double orc_wb_pressure(double e) { return wb_pressure(e); }
+double orc_wb_temperature(double e) { return wb_temperature(e); }
 
 /* Tabulated EOS: n nodes at ln(e) = lne0 + i*h; p and T at the nodes. */
 int orc_set_eos_table(orc_ctx *c, int n, double lne0, double h, const double *p, const double *T) {

@@ -91,6 +91,24 @@ def test_lqcd_table_eos_3d(oracle_mod):
     assert_parity(a, b, tau, label="table EOS")
 
 
+def test_wuppertal_budapest_eos_3d(oracle_mod):
+    """The reference's non-conformal EOS (Wuppertal-Budapest fits) on a 3+1D lattice."""
+    V = vah()
+    nx, ny, nz, steps = 20, 18, 10, 20
+    e = gaussian_ic(nx, ny, nz, dz=0.3, seed=6, peak=60.0)
+    n = nx * ny * nz
+    g = V.Vah(nx, ny, nz, 0.1, 0.1, 0.3, 0.02, 0.16, 0.5, eos_kind=V.EOS_WB)
+    o = oracle_mod.Oracle(nx, ny, nz, 0.1, 0.1, 0.3, 0.02, 0.16, 0.5, eos_wb=True)
+    g.set_primitives(e, *rest_frame(n), pl=e / 3)
+    o.set_primitives(e, *rest_frame(n), pl=e / 3)
+    g.step(steps)
+    g.synchronize()
+    o.step(steps)
+    assert_parity(g.state(), o.state(), o.time, label="WB EOS")
+    g.close()
+    o.close()
+
+
 def precision_errors(a, b):
     """Per-field max and RMS relative errors of the split-precision run a
     against the FP64 reference b (e and u^mu relative to u^tau, pi to e)."""

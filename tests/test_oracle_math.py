@@ -152,3 +152,28 @@ def test_regulator_floors(oracle_mod):
     ee, pp = np.array([-2.0]), np.array([-1.0])
     L.orc_regulate_cell(1.0, p(q), p(ee), p(pp), p(np.array([1.0, 0, 0, 0])))
     assert ee[0] == 1e-7 and pp[0] == 1e-7 and q[4] == 1e-7
+
+
+def test_wuppertal_budapest_fits_are_thermodynamically_consistent(oracle_mod):
+    """The reference's non-conformal EOS is two independent rational fits,
+    p(e) and T(e) (EquationOfState.cpp:31-76, :120-142).  With s = (e+p)/T,
+    thermodynamics fixes T ds/de = 1; the fits satisfy it to the fits'
+    accuracy (< 0.5 % over five decades of e), which a dropped term, a
+    wrong sign or a mis-scaled coefficient in either fit would break.  The
+    crossover (softest point p/e ~ 0.15 near T ~ 155 MeV) and the approach
+    towards the conformal 1/3 at high e are checked as well."""
+    L = oracle_mod.lib()
+    P, T = L.orc_wb_pressure, L.orc_wb_temperature
+    hbarc = 0.197326938
+    s = lambda x: (x + P(x)) / T(x)
+    for e in np.geomspace(0.05, 2000.0, 40):
+        h = e * 1e-5
+        ds = (s(e + h) - s(e - h)) / (2 * h)
+        assert abs(T(e) * ds - 1.0) < 5e-3, e
+    es = np.geomspace(0.05, 2000.0, 400)
+    ratio = np.array([P(e) / e for e in es])
+    soft = es[np.argmin(ratio)]
+    assert 0.13 < ratio.min() < 0.16 and 140.0 < T(soft) * hbarc * 1e3 < 175.0
+    assert 0.30 < P(2000.0) / 2000.0 < 1.0 / 3.0
+    Ts = np.array([T(e) for e in es])
+    assert np.all(np.diff(Ts) > 0)

@@ -221,11 +221,20 @@ template <int DIR, typename R, typename Emit>
 __device__ __forceinline__ void face_flux(double t, double uT_guess, const Buf &q, uint32_t f8, uint32_t c,
                                           uint32_t stp, Emit emit) {
     constexpr int NN = 9;  // TTT..PITN
-    R qL[NN], qR[NN];
+    // all loads of the nine fields first, so that they are in flight together
+    double raw[NN][4];
 #pragma unroll
     for (int n = 0; n < NN; ++n) {
         const uint32_t fo = n * f8;
-        R q0 = (R)ld(q, c - 2 * stp, fo), q1 = (R)ld(q, c - stp, fo), q2 = (R)ld(q, c, fo), q3 = (R)ld(q, c + stp, fo);
+        raw[n][0] = ld(q, c - 2 * stp, fo);
+        raw[n][1] = ld(q, c - stp, fo);
+        raw[n][2] = ld(q, c, fo);
+        raw[n][3] = ld(q, c + stp, fo);
+    }
+    R qL[NN], qR[NN];
+#pragma unroll
+    for (int n = 0; n < NN; ++n) {
+        R q0 = (R)raw[n][0], q1 = (R)raw[n][1], q2 = (R)raw[n][2], q3 = (R)raw[n][3];
         qL[n] = q1 + R(0.5) * slope(q0, q1, q2);
         qR[n] = q2 - R(0.5) * slope(q1, q2, q3);
     }
@@ -239,10 +248,18 @@ __device__ __forceinline__ void face_flux(double t, double uT_guess, const Buf &
     const R cR = (R)(0.5 * (vR - a)), cL = (R)(0.5 * (vL + a));
 #pragma unroll
     for (int n = 0; n < NN; ++n) emit(n, cR * qR[n] + cL * qL[n]);
+    double rest[NQ - NN][4];
 #pragma unroll
     for (int n = NN; n < NQ; ++n) {
         const uint32_t fo = n * f8;
-        R q0 = (R)ld(q, c - 2 * stp, fo), q1 = (R)ld(q, c - stp, fo), q2 = (R)ld(q, c, fo), q3 = (R)ld(q, c + stp, fo);
+        rest[n - NN][0] = ld(q, c - 2 * stp, fo);
+        rest[n - NN][1] = ld(q, c - stp, fo);
+        rest[n - NN][2] = ld(q, c, fo);
+        rest[n - NN][3] = ld(q, c + stp, fo);
+    }
+#pragma unroll
+    for (int n = NN; n < NQ; ++n) {
+        R q0 = (R)rest[n - NN][0], q1 = (R)rest[n - NN][1], q2 = (R)rest[n - NN][2], q3 = (R)rest[n - NN][3];
         R l = q1 + R(0.5) * slope(q0, q1, q2);
         R r = q2 - R(0.5) * slope(q1, q2, q3);
         emit(n, cR * r + cL * l);

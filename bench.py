@@ -51,7 +51,7 @@ def initial_state(V, w):
     return e
 
 
-def cpu_baseline(w, e, eos_table, seconds=20.0):
+def cpu_baseline(w, e, eos_table, seconds=15.0):
     """The CPU oracle (this project's C restatement of the reference CPU step,
     OpenMP over the host cores) on the same lattice for a few steps."""
     from oracle.oracle import Oracle
@@ -71,8 +71,24 @@ def cpu_baseline(w, e, eos_table, seconds=20.0):
         t_used += time.perf_counter() - a
         steps += 1
     o.close()
-    return {"value": n * steps / t_used, "unit": "cell-updates/s", "cores": cores, "kind": "port",
-            "sample": f"{steps} full-lattice steps of the same workload on the CPU oracle ({t_used:.1f} s)"}
+    out = {"value": n * steps / t_used, "unit": "cell-updates/s", "cores": cores, "kind": "port",
+           "sample": f"{steps} full-lattice steps of the same workload on the CPU oracle ({t_used:.1f} s)"}
+    # one core, as the reference's own single-threaded loops run: the middle
+    # 12 eta planes of the same initial state (same per-cell work), 2 steps
+    if w["nz"] > 12:
+        k0 = (w["nz"] - 12) // 2
+        sub = e.reshape(w["nz"], -1)[k0:k0 + 12].ravel()
+        m = sub.size
+        o1 = Oracle(w["nx"], w["ny"], 12, w["dx"], w["dx"], w["dz"], w["dt"], w["etabar"], w["t0"], threads=1,
+                    eos_table=eos_table)
+        o1.set_primitives(sub, np.ones(m), np.zeros(m), np.zeros(m), np.zeros(m), pl=sub / 3)
+        a = time.perf_counter()
+        o1.step(2)
+        t1 = time.perf_counter() - a
+        o1.close()
+        out["serial"] = {"value": m * 2 / t1, "cores": 1,
+                         "sample": f"2 steps of the middle 12 eta planes ({m} cells) on one core ({t1:.1f} s)"}
+    return out
 
 
 # ---------------------------------------------------------------------------
@@ -381,7 +397,7 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "fp32_split"], default="fp64",
                     help="fp32_split: stencil, fluxes and sources in float, Newton recovery in double")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--calibrate-copy", type=int, metavar="MIB", default=0,
                     help="only run a plain copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()

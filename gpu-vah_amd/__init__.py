@@ -144,6 +144,7 @@ def lib():
         L.vah_get_state.argtypes = [ctypes.c_void_p] + [_dp] * 5
         L.vah_get_time.argtypes = [ctypes.c_void_p, _dp, ctypes.POINTER(ctypes.c_longlong)]
         L.vah_reduce.argtypes = [ctypes.c_void_p, ctypes.POINTER(Stats)]
+        L.vah_validity.argtypes = [ctypes.c_void_p] + [_dp] * 4
         L.vah_profile_enable.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.vah_profile_read.argtypes = [ctypes.c_void_p, ctypes.POINTER(Profile)]
         L.vah_destroy.argtypes = [ctypes.c_void_p]
@@ -314,6 +315,12 @@ class Vah:
         u, up = np.empty(4 * n), np.empty(4 * n)
         _check(lib().vah_get_state(self.h, *[x.ctypes.data_as(_dp) for x in (q, e, p, u, up)]))
         return {"q": q.reshape(NQ, n), "e": e, "p": p, "u": u.reshape(4, n), "up": up.reshape(4, n)}
+
+    def validity(self):
+        """Knudsen numbers (theta_T, zDzu, theta) and R_pi after the last step."""
+        out = {k: np.empty(self.n) for k in ("kn_T", "kn_L", "kn", "r_pi")}
+        _check(lib().vah_validity(self.h, *[out[k].ctypes.data_as(_dp) for k in ("kn_T", "kn_L", "kn", "r_pi")]))
+        return out
 
     @property
     def time(self):

@@ -324,6 +324,61 @@ __global__ void reduce_kernel(const double *q, const double *e, const double *u,
         for (int k = 0; k < 4; ++k) out[blockIdx.x * 4 + k] = sm[k][0];
 }
 
+// First-order validity diagnostics (HydrodynamicValidity.cpp:47-247) with
+// the reference's arguments after a step: t_n, the conserved variables q^n of
+// the step's start, the new e, p, u and u^n as the previous velocity.
+// out[f*n + m] for f = kn_T, kn_L, kn, r_pi (m: interior cell, x fastest).
+__global__ void validity_kernel(const double *qn, const double *e, const double *p, const double *u,
+                                const double *up, long fs, int nx, int ny, int nz, long sy, long sz, double t,
+                                double dt, double etabar, double facX, double facY, double facZ, Eos eos,
+                                double *out) {
+    const long n = (long)nx * ny * nz;
+    const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= n) return;
+    const int ii = (int)(m % nx);
+    const int jj = (int)((m / nx) % ny);
+    const int kk = (int)(m / ((long)nx * ny));
+    const long s = (long)plane_of(kk, nz > 1) * sz + (long)(jj + 2) * sy + (ii + 2);
+    const double *ut_ = u, *ux_ = u + fs, *uy_ = u + 2 * fs, *un_ = u + 3 * fs;
+    const double ut = ut_[s], ux = ux_[s], uy = uy_[s], un = un_[s];
+    const double utp = up[s], unp = up[3 * fs + s];
+    const double dxux = (ux_[s + 1] - ux_[s - 1]) * facX;
+    const double dxut = (ut_[s + 1] - ut_[s - 1]) * facX, dxun = (un_[s + 1] - un_[s - 1]) * facX;
+    const double dyuy = (uy_[s + sy] - uy_[s - sy]) * facY;
+    const double dyut = (ut_[s + sy] - ut_[s - sy]) * facY, dyun = (un_[s + sy] - un_[s - sy]) * facY;
+    double dnut = 0.0, dnun = 0.0;
+    if (nz > 1) {
+        dnut = (ut_[s + sz] - ut_[s - sz]) * facZ;
+        dnun = (un_[s + sz] - un_[s - sz]) * facZ;
+    }
+    const double dtut = (ut - utp) / dt, dtun = (un - unp) / dt;
+    const double ut2 = ut * ut, un2 = un * un, t2 = t * t, t3 = t * t2;
+    const double Dut = ut * dtut + ux * dxut + uy * dyut + un * dnut + t * un * un;
+    const double dut = Dut - t * un * un;
+    const double dun = ut * dtun + ux * dxun + uy * dyun + un * dnun;
+    const double theta = ut / t + dtut + dxux + dyuy + dnun;
+    const double stt = -t * ut * un2 + (dtut - ut * dut) + (ut2 - 1) * theta / 3;
+    const double stn = -un * (2 * ut2 + t2 * un2) / (2 * t) + (dtun - dnut / t2) / 2 - (un * dut + ut * dun) / 2 +
+                       ut * un * theta / 3;
+    const double snn = -ut * (1 + 2 * t2 * un2) / t3 - dnun / t2 - un * dun + (1 / t2 + un2) * theta / 3;
+    const double F = 1 + ux * ux + uy * uy;
+    const double z0 = t * un / __builtin_sqrt(F), z3 = ut / t / __builtin_sqrt(F);
+    const double A = z0 * z0 * stt - 2 * t2 * z0 * z3 * stn + t2 * t2 * z3 * z3 * snn;
+    const double thetaT = 2 * theta / 3 + A;
+    const double zDzu = theta / 3 - A;
+    const double T = eos_T(eos, e[s]);
+    out[m] = 5 * etabar * fabs(thetaT) / T;
+    out[n + m] = 5 * etabar * fabs(zDzu) / T;
+    out[2 * n + m] = 5 * etabar * fabs(theta) / T;
+    const double pitt = qn[PITT * fs + s], pitx = qn[PITX * fs + s], pity = qn[PITY * fs + s];
+    const double pitn = qn[PITN * fs + s], pixx = qn[PIXX * fs + s], pixy = qn[PIXY * fs + s];
+    const double pixn = qn[PIXN * fs + s], piyy = qn[PIYY * fs + s], piyn = qn[PIYN * fs + s];
+    const double pinn = qn[PINN * fs + s];
+    const double pipi = pitt * pitt - 2 * pitx * pitx - 2 * pity * pity + pixx * pixx + 2 * pixy * pixy + piyy * piyy -
+                        2 * pitn * pitn * t2 + 2 * pixn * pixn * t2 + 2 * piyn * piyn * t2 + pinn * pinn * t2 * t2;
+    out[3 * n + m] = __builtin_sqrt(fabs(pipi)) / fabs(pt_hat(e[s], p[s], qn[PL * fs + s]));
+}
+
 }  // namespace
 
 extern "C" {
@@ -589,6 +644,27 @@ int vah_reduce(vah_ctx *c, vah_stats *out) {
     out->max_vT = mx_v;
     out->recovery_fallbacks = cnt[1];
     out->nan_cells = cnt[0];
+    return VAH_OK;
+}
+
+int vah_validity(vah_ctx *c, double *kn_T, double *kn_L, double *kn, double *r_pi) {
+    if (!c) return fail(VAH_ERR_ARG, "vah_validity: NULL context");
+    if (c->nstep < 1) return fail(VAH_ERR_STATE, "vah_validity: needs a completed step");
+    HIP_TRY(hipSetDevice(c->device));
+    const long n = (long)c->lat.nx * c->lat.ny * c->lat.nz;
+    // the stage-1 output set is free between steps: 4 interior-sized outputs fit in it
+    double *out = c->qset[c->qs];
+    const double t = c->hydro.t0 + (double)(c->nstep - 1) * c->lat.dt;
+    hipLaunchKernelGGL(validity_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, c->qset[c->qo],
+                       c->ebuf[c->ecur], c->pbuf[c->ecur], c->useq[c->uc], c->useq[c->up], c->fs, c->lat.nx,
+                       c->lat.ny, c->lat.nz, (long)c->ncx, (long)c->ncx * c->ncy, t, c->lat.dt, c->hydro.etabar,
+                       1 / c->lat.dx / 2, 1 / c->lat.dy / 2, 1 / c->lat.dz / 2, c->eos, out);
+    HIP_TRY(hipGetLastError());
+    double *dst[4] = {kn_T, kn_L, kn, r_pi};
+    for (int f = 0; f < 4; ++f)
+        if (dst[f])
+            HIP_TRY(hipMemcpyAsync(dst[f], out + (size_t)f * n, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
     return VAH_OK;
 }
 

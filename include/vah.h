@@ -144,6 +144,16 @@ int vah_get_time(const vah_ctx *ctx, double *tau, long long *nstep);
 int vah_reduce(vah_ctx *ctx, vah_stats *out);
 
 /* Per-launch HIP-event timing of the two stage kernels (off by default). */
+/* First-order validity diagnostics after a step, per interior cell: the
+ * Knudsen numbers 5 (eta/s) |theta_T| / T, 5 (eta/s) |zDzu| / T,
+ * 5 (eta/s) |theta| / T and the inverse Reynolds number sqrt|pi:pi| / |P_T hat|
+ * (knudsenNumberTaupiT / TaupiL / Taupi and Rpi of
+ * old_cpu-vah_code/src/HydrodynamicValidity.cpp:195-247, evaluated with the
+ * arguments of its call at FullyDiscreteKurganovTadmorScheme.cpp:791: the
+ * step's start time and conserved variables, the new e, p, u).  Any output
+ * may be NULL; valid until the next vah_step. */
+int vah_validity(vah_ctx *ctx, double *kn_T, double *kn_L, double *kn, double *r_pi);
+
 int vah_profile_enable(vah_ctx *ctx, int on);
 int vah_profile_read(vah_ctx *ctx, vah_profile *out);
 

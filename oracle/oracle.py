@@ -65,6 +65,8 @@ def lib():
         L.orc_nan_hits.argtypes = [ctypes.c_void_p]
         L.orc_nan_hits.restype = ctypes.c_long
         L.orc_get_state.argtypes = [ctypes.c_void_p] + [_dp] * 5
+        L.orc_validity.argtypes = [ctypes.c_void_p] + [_dp] * 4
+        L.orc_validity.restype = ctypes.c_int
         L.orc_set_eos_wb.argtypes = [ctypes.c_void_p]
         L.orc_set_eos_wb.restype = None
         for name in ("orc_wb_pressure", "orc_wb_temperature"):
@@ -179,6 +181,14 @@ class Oracle:
     @property
     def nan_hits(self):
         return lib().orc_nan_hits(self._handle())
+
+    def validity(self):
+        """Knudsen numbers (theta_T, zDzu, theta) and R_pi after the last step."""
+        n = self.n
+        out = {k: np.empty(n) for k in ("kn_T", "kn_L", "kn", "r_pi")}
+        if lib().orc_validity(self._handle(), *[_ptr(out[k]) for k in ("kn_T", "kn_L", "kn", "r_pi")]) != 0:
+            raise RuntimeError("validity needs a completed step")
+        return out
 
     def state(self):
         n = self.n

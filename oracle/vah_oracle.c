@@ -1042,6 +1042,73 @@ void orc_step(orc_ctx *c, int nsteps) {
 }
 
 double orc_time(const orc_ctx *c) { return c->t0 + (double)c->nstep * c->dt; }
+
+/* First-order validity diagnostics of checkValidityKernel
+ * (HydrodynamicValidity.cpp:47-247) with the arguments the reference passes
+ * after a step (FullyDiscreteKurganovTadmorScheme.cpp:791): the time t_n of
+ * the step's start, the conserved variables of that start (q^n: after the
+ * driver's swap they sit in Q), the new e, p, u and u^n as the previous
+ * velocity.  Per interior cell:
+ *   kn_T = 5 etabar |theta_T| / T,  kn_L = 5 etabar |zDzu| / T,
+ *   kn = 5 etabar |theta| / T,      r_pi = sqrt|pi:pi| / |P_T hat|
+ * (knudsenNumberTaupiT / TaupiL / Taupi and Rpi of the reference).  Returns
+ * -1 before the first step.  Any output may be NULL. */
+int orc_validity(const orc_ctx *c, double *kn_T, double *kn_L, double *kn, double *r_pi) {
+    if (c->nstep < 1) return -1;
+    const double t = c->t0 + (double)(c->nstep - 1) * c->dt;
+    const double dt = c->dt, etabar = c->etabar;
+    const long n = (long)c->nx * c->ny * c->nz;
+    const long sx = 1, sy = c->ncx, sz = (long)c->ncx * c->ncy;
+    const double facX = 1 / c->dx / 2, facY = 1 / c->dy / 2, facZ = 1 / c->dz / 2;
+    const double *ut_ = c->u[0], *ux_ = c->u[1], *uy_ = c->u[2], *un_ = c->u[3];
+#pragma omp parallel for num_threads(c->nthreads) schedule(static)
+    for (long m = 0; m < n; ++m) {
+        const long s = interior_index(c, m);
+        const double e_s = c->e[s], p_s = c->p[s];
+        const double ut = ut_[s], ux = ux_[s], uy = uy_[s], un = un_[s];
+        const double utp = c->up[0][s], uxp = c->up[1][s], uyp = c->up[2][s], unp = c->up[3][s];
+        const double dxut = (ut_[s + sx] - ut_[s - sx]) * facX, dxux = (ux_[s + sx] - ux_[s - sx]) * facX;
+        const double dxuy = (uy_[s + sx] - uy_[s - sx]) * facX, dxun = (un_[s + sx] - un_[s - sx]) * facX;
+        const double dyut = (ut_[s + sy] - ut_[s - sy]) * facY, dyux = (ux_[s + sy] - ux_[s - sy]) * facY;
+        const double dyuy = (uy_[s + sy] - uy_[s - sy]) * facY, dyun = (un_[s + sy] - un_[s - sy]) * facY;
+        const double dnut = (ut_[s + sz] - ut_[s - sz]) * facZ, dnux = (ux_[s + sz] - ux_[s - sz]) * facZ;
+        const double dnuy = (uy_[s + sz] - uy_[s - sz]) * facZ, dnun = (un_[s + sz] - un_[s - sz]) * facZ;
+        const double dtut = (ut - utp) / dt, dtux = (ux - uxp) / dt, dtuy = (uy - uyp) / dt, dtun = (un - unp) / dt;
+        const double ut2 = ut * ut, un2 = un * un, t2 = t * t, t3 = t * t2;
+        const double Dut = ut * dtut + ux * dxut + uy * dyut + un * dnut + t * un * un;
+        const double dut = Dut - t * un * un;
+        const double dux = ut * dtux + ux * dxux + uy * dyux + un * dnux;
+        const double duy = ut * dtuy + ux * dxuy + uy * dyuy + un * dnuy;
+        const double dun = ut * dtun + ux * dxun + uy * dyun + un * dnun;
+        (void)dux;
+        (void)duy;
+        const double theta = ut / t + dtut + dxux + dyuy + dnun;
+        const double stt = -t * ut * un2 + (dtut - ut * dut) + (ut2 - 1) * theta / 3;
+        const double stn = -un * (2 * ut2 + t2 * un2) / (2 * t) + (dtun - dnut / t2) / 2 - (un * dut + ut * dun) / 2 +
+                           ut * un * theta / 3;
+        const double snn = -ut * (1 + 2 * t2 * un2) / t3 - dnun / t2 - un * dun + (1 / t2 + un2) * theta / 3;
+        const double F = 1 + ux * ux + uy * uy;
+        const double z0 = t * un / sqrt(F), z3 = ut / t / sqrt(F);
+        const double A = z0 * z0 * stt - 2 * t2 * z0 * z3 * stn + t2 * t2 * z3 * z3 * snn;
+        const double thetaT = 2 * theta / 3 + A;
+        const double zDzu = theta / 3 - A;
+        const double T = eos_temperature(&c->eos, e_s);
+        if (kn_T) kn_T[m] = 5 * etabar * fabs(thetaT) / T;
+        if (kn_L) kn_L[m] = 5 * etabar * fabs(zDzu) / T;
+        if (kn) kn[m] = 5 * etabar * fabs(theta) / T;
+        if (r_pi) {
+            const double pl = c->Q[PL][s];
+            const double pitt = c->Q[PITT][s], pitx = c->Q[PITX][s], pity = c->Q[PITY][s], pitn = c->Q[PITN][s];
+            const double pixx = c->Q[PIXX][s], pixy = c->Q[PIXY][s], pixn = c->Q[PIXN][s], piyy = c->Q[PIYY][s];
+            const double piyn = c->Q[PIYN][s], pinn = c->Q[PINN][s];
+            const double pipi = pitt * pitt - 2 * pitx * pitx - 2 * pity * pity + pixx * pixx + 2 * pixy * pixy +
+                                piyy * piyy - 2 * pitn * pitn * t2 + 2 * pixn * pixn * t2 + 2 * piyn * piyn * t2 +
+                                pinn * pinn * t2 * t2;
+            r_pi[m] = sqrt(fabs(pipi)) / fabs(pt_hat(e_s, p_s, pl));
+        }
+    }
+    return 0;
+}
 long orc_newton_fail(const orc_ctx *c) { return c->newton_fail; }
 long orc_nan_hits(const orc_ctx *c) { return c->nan_hits; }
 

@@ -91,6 +91,31 @@ def test_lqcd_table_eos_3d(oracle_mod):
     assert_parity(a, b, tau, label="table EOS")
 
 
+def test_validity_diagnostics_3d(oracle_mod):
+    """Knudsen and inverse Reynolds maps after 15 steps of a flowing 3+1D state:
+    the GPU's validity diagnostics against the oracle's, per cell within 1e-10
+    of each map's largest value."""
+    V = vah()
+    nx, ny, nz = 22, 20, 9
+    e = gaussian_ic(nx, ny, nz, dz=0.3, seed=12, peak=80.0)
+    n = nx * ny * nz
+    g = V.Vah(nx, ny, nz, 0.1, 0.1, 0.3, 0.01, 0.2, 0.5)
+    o = oracle_mod.Oracle(nx, ny, nz, 0.1, 0.1, 0.3, 0.01, 0.2, 0.5)
+    g.set_primitives(e, *rest_frame(n), pl=e / 3)
+    o.set_primitives(e, *rest_frame(n), pl=e / 3)
+    g.step(15)
+    g.synchronize()
+    o.step(15)
+    a, b = g.validity(), o.validity()
+    for k in ("kn_T", "kn_L", "kn", "r_pi"):
+        scale = np.max(np.abs(b[k]))
+        err = np.max(np.abs(a[k] - b[k])) / scale
+        print(k, "max", scale, "worst", err)
+        assert scale > 0 and err < 1e-10, k
+    g.close()
+    o.close()
+
+
 def test_wuppertal_budapest_eos_3d(oracle_mod):
     """The reference's non-conformal EOS (Wuppertal-Budapest fits) on a 3+1D lattice."""
     V = vah()

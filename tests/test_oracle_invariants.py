@@ -151,3 +151,27 @@ def test_boost_invariance(oracle_mod):
         sl = slice(k * nx * ny, (k + 1) * nx * ny)
         np.testing.assert_array_equal(b["e"][sl], a["e"])
         np.testing.assert_array_equal(b["q"][:, sl], a["q"])
+
+
+def test_validity_diagnostics_in_bjorken_flow(oracle_mod):
+    """Bjorken flow (uniform lattice at rest): theta = 1/tau, the transverse
+    expansion theta_T = 2 theta/3 + A vanishes (A = tau^2 sigma^etaeta =
+    -2/(3 tau)) and zDzu = 1/tau, so Kn_T = 0 and Kn_L = Kn = 5 (eta/s)/(tau T).
+    The diagnostics use the start time of the last step and T of the new e
+    (the reference's call after the step).  pi_perp stays 0, so R_pi = 0."""
+    nx, ny, dt, t0 = 5, 4, 0.01, 0.5
+    n = nx * ny
+    e = np.full(n, E0)
+    o = oracle_mod.Oracle(nx, ny, 1, 0.1, 0.1, 0.1, dt, ETABAR, t0)
+    o.set_primitives(e, *rest_frame(n), pl=e / 3)
+    with pytest.raises(RuntimeError):
+        o.validity()
+    o.step(7)
+    v = o.validity()
+    tn = t0 + 6 * dt
+    T = T_single(o.state()["e"][0])
+    assert np.all(np.abs(v["kn_T"]) < 1e-12)
+    assert np.allclose(v["kn_L"], 5 * ETABAR / (tn * T), rtol=1e-12)
+    assert np.allclose(v["kn"], 5 * ETABAR / (tn * T), rtol=1e-12)
+    assert np.all(v["r_pi"] < 1e-12)
+    o.close()

@@ -404,9 +404,7 @@ class Batch:
               max_steps=0, tag=0, raise_on_full=True):
         """Admit one event; returns its slot, or None when the arena or the
         slot table is full and raise_on_full is False."""
-        n = nx * ny * nz
-        if ut is None:
-            ut, ux, uy, un = np.ones(n), np.zeros(n), np.zeros(n), np.zeros(n)
+        # ut = ux = uy = un = None: the rest frame (set on the device)
         ev = BatchEvent(nx, ny, nz, dx, dy, dz, dt, t0, etabar, max_steps, tag)
         args = [_arr(x) for x in (e, pl, ut, ux, uy, un, pi)]
         slot = ctypes.c_int(-1)

@@ -100,6 +100,16 @@ struct vah_batch {
     // kernels
     int variant;
     std::map<int, hipGraphExec_t> graphs;  // key: (has2d, has3d)
+    // pinned staging buffers for admissions, reused round-robin once their
+    // copies have completed
+    struct Staging {
+        char *host = nullptr;
+        size_t cap = 0;
+        hipEvent_t done = nullptr;
+        bool pending = false;
+    };
+    Staging stage[4];
+    int stage_next;
 };
 
 namespace {
@@ -293,6 +303,83 @@ int graph_for(vah_batch *b, bool has2, bool has3, hipGraphExec_t *out) {
 
 bool slot_ok(const vah_batch *b, int slot) { return b && slot >= 0 && slot < b->max_events && b->events[slot].used; }
 
+// Initial state of an admitted event from compact interior arrays
+// [e | P_L? | u^tau u^x u^y u^eta? | pi (10)?] (each n values): the map of
+// setConservedVariables (DynamicalVariables.cpp:197-287) -- without P_L the
+// reference's 0.0154483 e, without u the rest frame, without pi zero -- into
+// the padded level-0 buffers (q, e, p, u^n and u^{n-1} = u^n).
+__global__ void init_event_kernel(const double *in, int has_pl, int has_u, int has_pi, int nx, int ny, int nz,
+                                  long sy, long sz, long fs, double t, Eos eos, double *q, double *e, double *p,
+                                  double *u0, double *u1) {
+    const long n = (long)nx * ny * nz;
+    const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= n) return;
+    const int ii = (int)(m % nx);
+    const int jj = (int)((m / nx) % ny);
+    const int kk = (int)(m / ((long)nx * ny));
+    const long s = (long)plane_of(kk, nz > 1) * sz + (long)(jj + 2) * sy + (ii + 2);
+    const double *col = in + n;
+    const double eeq = in[m];
+    double plv = 0.0154483 * eeq;
+    if (has_pl) {
+        plv = col[m];
+        col += n;
+    }
+    double vt = 1.0, vx = 0.0, vy = 0.0, vn = 0.0;
+    if (has_u) {
+        vt = col[m];
+        vx = col[n + m];
+        vy = col[2 * n + m];
+        vn = col[3 * n + m];
+        col += 4 * n;
+    }
+    double pis[10];
+#pragma unroll
+    for (int f = 0; f < 10; ++f) pis[f] = has_pi ? col[f * n + m] : 0.0;
+    const double pt = 0.5 * (eeq - plv);
+    const double DP = plv - pt;
+    const double F = 1 + (vx * vx + vy * vy);
+    const double Ltt = DP * t * t * vn * vn / F;
+    const double Ltn = DP * vt * vn / F;
+    q[TTT * fs + s] = (eeq + pt) * vt * vt - pt + Ltt + pis[0];
+    q[TTX * fs + s] = (eeq + pt) * vt * vx + pis[1];
+    q[TTY * fs + s] = (eeq + pt) * vt * vy + pis[2];
+    q[TTN * fs + s] = (eeq + pt) * vt * vn + Ltn + pis[3];
+    q[PL * fs + s] = plv;
+#pragma unroll
+    for (int f = 0; f < 10; ++f) q[(PITT + f) * fs + s] = pis[f];
+    e[s] = eeq;
+    p[s] = eos_p(eos, eeq);
+    const double uv[4] = {vt, vx, vy, vn};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        u0[k * fs + s] = uv[k];
+        u1[k * fs + s] = uv[k];
+    }
+}
+
+// a pinned staging buffer of at least `bytes`, free of pending copies
+int staging(vah_batch *b, size_t bytes, vah_batch::Staging **out) {
+    vah_batch::Staging &st = b->stage[b->stage_next];
+    b->stage_next = (b->stage_next + 1) % 4;
+    if (st.pending) {
+        HIP_TRY(hipEventSynchronize(st.done));
+        st.pending = false;
+    }
+    if (st.cap < bytes) {
+        if (st.host) (void)hipHostFree(st.host);
+        st.host = nullptr;
+        st.cap = 0;
+        const size_t cap = bytes + bytes / 4;
+        if (hipHostMalloc((void **)&st.host, cap, hipHostMallocDefault) != hipSuccess)
+            return fail(VAH_ERR_NOMEM, "vah_batch_admit: %.1f MB of pinned staging", cap / 1e6);
+        st.cap = cap;
+    }
+    if (!st.done) HIP_TRY(hipEventCreateWithFlags(&st.done, hipEventDisableTiming));
+    *out = &st;
+    return VAH_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -338,6 +425,10 @@ void vah_batch_destroy(vah_batch *b) {
     (void)hipFree(b->powf_fix);
     (void)hipFree(b->eos_dev);
     if (b->dyn_host) (void)hipHostFree(b->dyn_host);
+    for (auto &st : b->stage) {
+        if (st.host) (void)hipHostFree(st.host);
+        if (st.done) (void)hipEventDestroy(st.done);
+    }
     if (b->own_stream && b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
@@ -421,8 +512,10 @@ int vah_batch_create(const vah_batch_options *opt_in, vah_batch **out) {
 
 int vah_batch_admit(vah_batch *b, const vah_batch_event *d, const double *e, const double *pl, const double *ut,
                     const double *ux, const double *uy, const double *un, const double *pi, int *slot) {
-    if (!b || !d || !e || !ut || !ux || !uy || !un || !slot) return fail(VAH_ERR_ARG, "vah_batch_admit: NULL argument");
+    if (!b || !d || !e || !slot) return fail(VAH_ERR_ARG, "vah_batch_admit: NULL argument");
     *slot = -1;
+    const int nu = (ut != nullptr) + (ux != nullptr) + (uy != nullptr) + (un != nullptr);
+    if (nu != 0 && nu != 4) return fail(VAH_ERR_ARG, "vah_batch_admit: give all four u^mu arrays or none");
     if (d->nx <= 0 || d->ny <= 0 || d->nz <= 0 || !(d->dx > 0) || !(d->dy > 0) || !(d->dz > 0) || !(d->dt > 0))
         return fail(VAH_ERR_ARG, "vah_batch_admit: lattice sizes and spacings must be positive");
     if (!(d->t0 > 0) || !(d->etabar > 0)) return fail(VAH_ERR_ARG, "vah_batch_admit: need t0 > 0 and eta/s > 0");
@@ -435,56 +528,76 @@ int vah_batch_admit(vah_batch *b, const vah_batch_event *d, const double *e, con
     if (s < 0) return fail(VAH_ERR_FULL, "vah_batch_admit: all %d event slots are taken", b->max_events);
     if ((double)vah_host::Layout(d->nx, d->ny, d->nz).fs * NQ * 8 >= 4294967296.0)
         return fail(VAH_ERR_ARG, "vah_batch_admit: a 15-field set of this lattice exceeds 4 GB");
+    HIP_TRY(hipSetDevice(b->device));
+    const long n = (long)d->nx * d->ny * d->nz;
+    const int nin = 1 + (pl ? 1 : 0) + (nu ? 4 : 0) + (pi ? 10 : 0);
+    const size_t in_bytes = sizeof(double) * (size_t)nin * n;
+    const size_t desc_off = (in_bytes + 255) / 256 * 256;
+    const size_t dyn_off = desc_off + (sizeof(BatchEvent) + 255) / 256 * 256;
+    vah_batch::Staging *st = nullptr;
+    int rc;
+    if ((rc = staging(b, dyn_off + sizeof(BatchDyn), &st)) != VAH_OK) return rc;
     const size_t bytes = event_bytes(d->nx, d->ny, d->nz);
     size_t off = 0;
     if (!arena_alloc(b, bytes, &off))
         return fail(VAH_ERR_BUDGET, "vah_batch_admit: %.1f MB do not fit (%.1f of %.1f MB in use)", bytes / 1e6,
                     b->used_bytes / 1e6, b->arena_bytes / 1e6);
-    HIP_TRY(hipSetDevice(b->device));
-    HIP_TRY(hipStreamSynchronize(b->stream));
     HostEvent &h = b->events[s];
     h.desc = *d;
     h.lay = vah_host::Layout(d->nx, d->ny, d->nz);
     h.offset = off;
     h.bytes = bytes;
     fill_descriptor(b, h);
-
-    // initial state: q^0 in slab 0, e/p level 0, u^0 = u^{-1} in slots 0 and 1
-    const long n = h.lay.interior();
-    std::vector<double> q((size_t)NQ * n), ev(n), pv(n), u((size_t)4 * n);
-    vah_host::prims_to_conserved(d->t0, b->heos, n, e, pl, ut, ux, uy, un, pi, q.data(), ev.data(), pv.data(),
-                                 u.data());
-    std::vector<double> img;
-    auto put = [&](double *dst, int nf, const double *src) -> int {
-        h.lay.pad(nf, src, img);
-        HIP_TRY(hipMemcpyAsync(dst, img.data(), sizeof(double) * nf * h.lay.fs, hipMemcpyHostToDevice, b->stream));
-        HIP_TRY(hipStreamSynchronize(b->stream));
-        return VAH_OK;
-    };
-    int rc;
     auto undo = [&](int code) {
         arena_free(b, h.offset, h.bytes);
         h.used = false;
         return code;
     };
-    if ((rc = put(h.dev.q[0], NQ, q.data())) != VAH_OK) return undo(rc);
-    if ((rc = put(h.dev.e[0], 1, ev.data())) != VAH_OK) return undo(rc);
-    if ((rc = put(h.dev.p[0], 1, pv.data())) != VAH_OK) return undo(rc);
-    if ((rc = put(h.dev.u[0], 4, u.data())) != VAH_OK) return undo(rc);
-    if ((rc = put(h.dev.u[1], 4, u.data())) != VAH_OK) return undo(rc);
-    const int blocks = (int)((n + 255) / 256);
-    hipLaunchKernelGGL(ghost_fill_kernel, dim3(blocks), dim3(256), 0, b->stream, h.dev.q[0], h.dev.e[0], h.dev.p[0],
-                       h.dev.u[0], h.lay.fs, d->nx, d->ny, d->nz, (long)h.lay.ncx, (long)h.lay.ncx * h.lay.ncy,
-                       d->nz > 1 ? 1 : 0);
-    if (hipGetLastError() != hipSuccess) return undo(fail(VAH_ERR_HIP, "vah_batch_admit: ghost fill launch"));
+
+    // compact inputs into pinned memory, then everything else on the device:
+    // the inputs land in the event's own q* and q^{n+1} sets (free until its
+    // first step), a kernel maps them to the padded level-0 state
+    double *col = (double *)st->host;
+    auto put = [&](const double *src) {
+        memcpy(col, src, sizeof(double) * n);
+        col += n;
+    };
+    put(e);
+    if (pl) put(pl);
+    if (nu) {
+        put(ut);
+        put(ux);
+        put(uy);
+        put(un);
+    }
+    if (pi) {
+        memcpy(col, pi, sizeof(double) * 10 * n);
+        col += 10 * n;
+    }
     BatchDyn dyn;
     memset(&dyn, 0, sizeof dyn);
     dyn.live = 1;
     dyn.state = EV_LIVE;
-    if (hipMemcpyAsync(b->ev_dev + s, &h.dev, sizeof(BatchEvent), hipMemcpyHostToDevice, b->stream) != hipSuccess ||
-        hipMemcpyAsync(b->dyn_dev + s, &dyn, sizeof dyn, hipMemcpyHostToDevice, b->stream) != hipSuccess ||
-        hipStreamSynchronize(b->stream) != hipSuccess)
+    memcpy(st->host + desc_off, &h.dev, sizeof(BatchEvent));
+    memcpy(st->host + dyn_off, &dyn, sizeof(BatchDyn));
+    double *dev_in = h.dev.q[1];  // 2 x 15 fields of fs >= nin * n values
+    const int blocks = (int)((n + 255) / 256);
+    if (hipMemcpyAsync(dev_in, st->host, in_bytes, hipMemcpyHostToDevice, b->stream) != hipSuccess)
+        return undo(fail(VAH_ERR_HIP, "vah_batch_admit: input upload"));
+    hipLaunchKernelGGL(init_event_kernel, dim3(blocks), dim3(256), 0, b->stream, dev_in, pl ? 1 : 0, nu ? 1 : 0,
+                       pi ? 1 : 0, d->nx, d->ny, d->nz, (long)h.lay.ncx, (long)h.lay.ncx * h.lay.ncy, h.lay.fs, d->t0,
+                       b->eos, h.dev.q[0], h.dev.e[0], h.dev.p[0], h.dev.u[0], h.dev.u[1]);
+    hipLaunchKernelGGL(ghost_fill_kernel, dim3(blocks), dim3(256), 0, b->stream, h.dev.q[0], h.dev.e[0], h.dev.p[0],
+                       h.dev.u[0], h.lay.fs, d->nx, d->ny, d->nz, (long)h.lay.ncx, (long)h.lay.ncx * h.lay.ncy,
+                       d->nz > 1 ? 1 : 0);
+    if (hipGetLastError() != hipSuccess) return undo(fail(VAH_ERR_HIP, "vah_batch_admit: initial-state launch"));
+    if (hipMemcpyAsync(b->ev_dev + s, st->host + desc_off, sizeof(BatchEvent), hipMemcpyHostToDevice, b->stream) !=
+            hipSuccess ||
+        hipMemcpyAsync(b->dyn_dev + s, st->host + dyn_off, sizeof(BatchDyn), hipMemcpyHostToDevice, b->stream) !=
+            hipSuccess ||
+        hipEventRecord(st->done, b->stream) != hipSuccess)
         return undo(fail(VAH_ERR_HIP, "vah_batch_admit: descriptor upload"));
+    st->pending = true;
     h.used = true;
     b->lists_dirty = true;
     *slot = s;

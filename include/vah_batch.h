@@ -95,8 +95,10 @@ size_t vah_batch_event_bytes(int nx, int ny, int nz);
 int vah_batch_create(const vah_batch_options *opt, vah_batch **out);
 
 /* Admits one event from primitives (setConservedVariables semantics as in
- * vah_set_primitives: pl NULL -> 0.0154483 e, pi NULL -> 0).  Waits for
- * steps in flight.  *slot receives the event's slot. */
+ * vah_set_primitives: pl NULL -> 0.0154483 e, pi NULL -> 0; here also
+ * ut = ux = uy = un = NULL -> the rest frame).  The inputs are copied before
+ * the call returns; the upload and the initial-state map run on the device,
+ * ordered after steps already enqueued.  *slot receives the event's slot. */
 int vah_batch_admit(vah_batch *b, const vah_batch_event *ev, const double *e, const double *pl, const double *ut,
                     const double *ux, const double *uy, const double *un, const double *pi, int *slot);
 

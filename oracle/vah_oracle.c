@@ -39,6 +39,16 @@
  *   - initial P_L = 0.0154483 e unless the caller supplies P_L
  *     (DynamicalVariables.cpp:248).
  *
+ * Pins (tests/): source terms, axis sources, EOS, fits, limiter and face
+ * extrapolation agree bit for bit with the reference's own files built by
+ * oracle/Makefile (target ref); the Newton recovery, regulator, RK2
+ * composition and ghost fill are pinned by round trips, closed forms, the
+ * conformal Bjorken solution and lattice symmetries; the Wuppertal-Budapest
+ * fits by thermodynamic consistency (T ds/de = 1).  orc_validity is pinned on
+ * Bjorken flow (theta, the longitudinal projection A, the Knudsen numbers);
+ * its u^eta-dependent terms of A and its R_pi with nonzero pi are parity
+ * unpinned (the reference file including them needs libconfig to build).
+ *
  * The tabulated equation of state (EOS_TABLE) is not in the reference: it is
  * this project's lattice-QCD-like EOS, looked up exactly as the product does.
  * EOS_WB is the reference's non-conformal branch (the Wuppertal-Budapest

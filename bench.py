@@ -406,6 +406,7 @@ def main():
     ap.add_argument("--flux-mode", choices=["shared", "per_cell"], default="shared")
     ap.add_argument("--precision", choices=["fp64", "fp32_split"], default="fp64",
                     help="fp32_split: stencil, fluxes and sources in float, Newton recovery in double")
+    ap.add_argument("--eta-chunk", type=int, default=0, help="eta planes per workgroup (0: library default)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--calibrate-copy", type=int, metavar="MIB", default=0,
@@ -433,7 +434,7 @@ def main():
     mode = V.FLUX_SHARED if args.flux_mode == "shared" else V.FLUX_PER_CELL
     g = V.Vah(w["nx"], w["ny"], w["nz"], w["dx"], w["dx"], w["dz"], w["dt"], w["etabar"], w["t0"],
               eos_table=eos_table, flux_mode=mode,
-              precision=V.FP32_SPLIT if args.precision == "fp32_split" else V.FP64)
+              precision=V.FP32_SPLIT if args.precision == "fp32_split" else V.FP64, eta_chunk=args.eta_chunk)
     g.set_primitives(e, np.ones(n), np.zeros(n), np.zeros(n), np.zeros(n), pl=e / 3)
     g.step(args.warmup)
     g.synchronize()

@@ -51,7 +51,7 @@ class EosTable(ctypes.Structure):
 
 class Options(ctypes.Structure):
     _fields_ = [("eos", ctypes.c_int), ("table", EosTable), ("precision", ctypes.c_int), ("flux_mode", ctypes.c_int),
-                ("device", ctypes.c_int), ("stream", ctypes.c_void_p)]
+                ("device", ctypes.c_int), ("stream", ctypes.c_void_p), ("eta_chunk", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):
@@ -257,7 +257,7 @@ class Vah:
     """One lattice evolved on the GPU through the C ABI."""
 
     def __init__(self, nx, ny, nz, dx, dy, dz, dt, etabar, t0, eos_table=None, flux_mode=FLUX_SHARED,
-                 precision=FP64, device=0, nt=0, freezeout_T_GeV=0.155, eos_kind=None):
+                 precision=FP64, device=0, nt=0, freezeout_T_GeV=0.155, eos_kind=None, eta_chunk=0):
         """eos_table (lne0, h, p, T) selects the tabulated EOS; eos_kind=EOS_WB the
         reference's Wuppertal-Budapest fits; neither: the conformal EOS."""
         L = lib()
@@ -270,6 +270,7 @@ class Vah:
         opt.flux_mode = flux_mode
         opt.precision = precision
         opt.device = device
+        opt.eta_chunk = eta_chunk
         self._keep = []
         if eos_table is not None:
             lne0, h, p, T = eos_table

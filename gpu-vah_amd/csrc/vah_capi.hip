@@ -412,6 +412,8 @@ int vah_create(const vah_lattice *lat, const vah_hydro *hydro, const vah_options
         return fail(VAH_ERR_ARG, "vah_create: unknown precision %d", opt.precision);
     if (opt.precision == VAH_FP32_SPLIT && opt.flux_mode != VAH_FLUX_SHARED)
         return fail(VAH_ERR_ARG, "vah_create: the split-precision form uses shared faces only");
+    if (opt.eta_chunk < 0 || opt.eta_chunk > 256)
+        return fail(VAH_ERR_ARG, "vah_create: eta_chunk must be in [0, 256]");
     if (opt.flux_mode != VAH_FLUX_SHARED && opt.flux_mode != VAH_FLUX_PER_CELL)
         return fail(VAH_ERR_ARG, "vah_create: unknown flux mode %d", opt.flux_mode);
     if (opt.eos == VAH_EOS_TABLE && (opt.table.n < 2 || !opt.table.p || !opt.table.T || !(opt.table.h > 0)))
@@ -491,7 +493,7 @@ int vah_create(const vah_lattice *lat, const vah_hydro *hydro, const vah_options
     // launch geometry
     c->tiles_x = (lat->nx + kTileX - 1) / kTileX;
     c->tiles_y = (lat->ny + kTileY - 1) / kTileY;
-    c->kz = lat->nz > 1 ? std::min(8, lat->nz) : 1;
+    c->kz = lat->nz > 1 ? std::min(opt.eta_chunk > 0 ? opt.eta_chunk : 4, lat->nz) : 1;
     c->zchunks = (lat->nz + c->kz - 1) / c->kz;
     c->nblocks = c->tiles_x * c->tiles_y * c->zchunks;
     c->grid = (c->nblocks + 7) / 8 * 8;

@@ -62,6 +62,7 @@ constexpr int kNumVariants = (int)(sizeof kVariants / sizeof kVariants[0]);
 
 struct HostEvent {
     bool used = false;
+    bool stopped = false;  // seen stopped by a poll: its tiles leave the lists
     vah_batch_event desc;
     vah_host::Layout lay;
     size_t offset = 0, bytes = 0;
@@ -219,7 +220,7 @@ int upload_lists(vah_batch *b) {
     std::vector<int> act;
     for (int s = 0; s < b->max_events; ++s) {
         const HostEvent &h = b->events[s];
-        if (!h.used) continue;
+        if (!h.used || h.stopped) continue;  // a stopped event's fields stay put without its tiles
         act.push_back(s);
         const int nt = tiles_of(b, h);
         std::vector<int2> &dst = h.desc.nz > 1 ? t3 : t2;
@@ -599,6 +600,7 @@ int vah_batch_admit(vah_batch *b, const vah_batch_event *d, const double *e, con
         return undo(fail(VAH_ERR_HIP, "vah_batch_admit: descriptor upload"));
     st->pending = true;
     h.used = true;
+    h.stopped = false;
     b->lists_dirty = true;
     *slot = s;
     return VAH_OK;
@@ -641,8 +643,12 @@ int vah_batch_poll(vah_batch *b, vah_batch_status *out, int cap, int *n) {
     HIP_TRY(hipStreamSynchronize(b->stream));
     int k = 0;
     for (int s = 0; s < b->max_events; ++s) {
-        const HostEvent &h = b->events[s];
+        HostEvent &h = b->events[s];
         if (!h.used) continue;
+        if (b->dyn_host[s].state != EV_LIVE && !h.stopped) {
+            h.stopped = true;  // drop its tiles from the lists at the next step
+            b->lists_dirty = true;
+        }
         if (k < cap) {
             const BatchDyn &d = b->dyn_host[s];
             vah_batch_status &o = out[k];
@@ -699,6 +705,7 @@ int vah_batch_retire(vah_batch *b, int slot) {
     HIP_TRY(hipStreamSynchronize(b->stream));
     arena_free(b, h.offset, h.bytes);
     h.used = false;
+    h.stopped = false;
     b->lists_dirty = true;
     return VAH_OK;
 }

@@ -103,7 +103,7 @@ __global__ __launch_bounds__(NTHREADS, BLOCKS) void batch_stage_kernel(const Bat
         P.t_rec = t + E.dt;
         P.counters = d->counters;
         P.emax = STAGE2 ? &d->emax : nullptr;
-        stage_tile<D3, STAGE2, true, R, TX, TY>(P, tile, lds.Hs, lds.Hz);
+        stage_tile<D3, STAGE2, true, R, TX, TY>(P, tile, lds.Hs, lds.Hz, &lds.Zr[0][0][0]);
         __syncthreads();  // the next tile rewrites the face buffers
     }
 }

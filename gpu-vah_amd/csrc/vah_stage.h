@@ -217,19 +217,16 @@ __device__ __forceinline__ Views make_views(const StageParams &P) {
 // (HalfSiteExtrapolation.cpp:12-26); emit(n, H_n) receives the 15 components
 // in order.  The nine components the Newton solve needs stay in registers,
 // the other six are reconstructed after it, which keeps the live state small.
-template <int DIR, typename R, typename Emit>
-__device__ __forceinline__ void face_flux(double t, double uT_guess, const Buf &q, uint32_t f8, uint32_t c,
-                                          uint32_t stp, Emit emit) {
+template <int DIR, typename R, typename Load, typename Emit>
+__device__ __forceinline__ void face_flux_from(double t, double uT_guess, Load load, Emit emit) {
     constexpr int NN = 9;  // TTT..PITN
-    // all loads of the nine fields first, so that they are in flight together
+    // all loads of the nine fields first, so that they are in flight together;
+    // load(n, j) gives field n at stencil position j (0..3: c-2st .. c+st)
     double raw[NN][4];
 #pragma unroll
     for (int n = 0; n < NN; ++n) {
-        const uint32_t fo = n * f8;
-        raw[n][0] = ld(q, c - 2 * stp, fo);
-        raw[n][1] = ld(q, c - stp, fo);
-        raw[n][2] = ld(q, c, fo);
-        raw[n][3] = ld(q, c + stp, fo);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) raw[n][j] = load(n, j);
     }
     R qL[NN], qR[NN];
 #pragma unroll
@@ -251,11 +248,8 @@ __device__ __forceinline__ void face_flux(double t, double uT_guess, const Buf &
     double rest[NQ - NN][4];
 #pragma unroll
     for (int n = NN; n < NQ; ++n) {
-        const uint32_t fo = n * f8;
-        rest[n - NN][0] = ld(q, c - 2 * stp, fo);
-        rest[n - NN][1] = ld(q, c - stp, fo);
-        rest[n - NN][2] = ld(q, c, fo);
-        rest[n - NN][3] = ld(q, c + stp, fo);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) rest[n - NN][j] = load(n, j);
     }
 #pragma unroll
     for (int n = NN; n < NQ; ++n) {
@@ -264,6 +258,18 @@ __device__ __forceinline__ void face_flux(double t, double uT_guess, const Buf &
         R r = q2 - R(0.5) * slope(q1, q2, q3);
         emit(n, cR * r + cL * l);
     }
+}
+
+// Flux through the face between cells c-st and c (byte offsets).  The
+// left/right states are the reference's half-cell extrapolations
+// (HalfSiteExtrapolation.cpp:12-26); emit(n, H_n) receives the 15 components
+// in order.  The nine components the Newton solve needs stay in registers,
+// the other six are loaded and reconstructed after it.
+template <int DIR, typename R, typename Emit>
+__device__ __forceinline__ void face_flux(double t, double uT_guess, const Buf &q, uint32_t f8, uint32_t c,
+                                          uint32_t stp, Emit emit) {
+    face_flux_from<DIR, R>(
+        t, uT_guess, [&](int n, int j) { return ld(q, c + (uint32_t)j * stp - 2u * stp, n * f8); }, emit);
 }
 
 __device__ __forceinline__ double uT_of(const Buf &u, uint32_t f8, uint32_t c) {
@@ -422,7 +428,15 @@ __device__ __forceinline__ void count_flags(const StageParams &P, bool nan, bool
 // touches.  R is the arithmetic type of the stencil, fluxes and sources; the
 // tile is TX x TY cells (P.tiles_x / P.tiles_y must be counted with them).
 template <bool D3, bool STAGE2, bool SHARED, typename R, int TX = kTileX, int TY = kTileY>
-__device__ __forceinline__ void stage_tile(const StageParams &P, int tile, R (*Hs)[NTHREADS], R (*Hz)[NTHREADS]) {
+__device__ __forceinline__ void stage_tile(const StageParams &P, int tile, R (*Hs)[NTHREADS], R (*Hz)[NTHREADS],
+                                           R *zring) {
+    // split precision in 3+1D: the eta-face stencil of this lane's column is
+    // kept in LDS for three planes (lane-private slots), so each plane's 15
+    // fields are read from memory once by the eta faces instead of four times
+    constexpr bool ZRING = D3 && SHARED && sizeof(R) == 4;
+    auto zr = [&](int plane, int n) -> R & {  // ring slot of a plane (planes >= -1) and field
+        return zring[(((plane + 3) % 3) * NQ + n) * NTHREADS + (int)threadIdx.x];
+    };
     static_assert(tile_fits<TX, TY>(), "tile does not fit one workgroup");
     const int tpl = P.tiles_x * P.tiles_y;
     const int zc = tile / tpl;
@@ -506,15 +520,42 @@ __device__ __forceinline__ void stage_tile(const StageParams &P, int tile, R (*H
             __syncthreads();
             // ---- eta faces: solved by the cell's own lane, the upper one carried to the next plane
             if (D3 && cell_ok) {
-                if (kk == k0)
-                    face_flux<3, R>(t, uT_of(V.ucur, f8, s - sz8), V.qin, f8, s, sz8,
-                                    [&](int n, R h) { Hz[n][tid] = h; });
+                if (kk == k0) {
+                    if constexpr (ZRING) {
+                        // lower face: planes k0-2 .. k0+1 from memory, the ring keeps k0-1 .. k0+1
+                        face_flux_from<3, R>(
+                            t, uT_of(V.ucur, f8, s - sz8),
+                            [&](int n, int j) {
+                                const double v = ld(V.qin, s + (uint32_t)j * sz8 - 2u * sz8, n * f8);
+                                if (j >= 1) zr(kk + j - 2, n) = (R)v;
+                                return v;
+                            },
+                            [&](int n, R h) { Hz[n][tid] = h; });
+                    } else {
+                        face_flux<3, R>(t, uT_of(V.ucur, f8, s - sz8), V.qin, f8, s, sz8,
+                                        [&](int n, R h) { Hz[n][tid] = h; });
+                    }
+                }
                 R Sa[4];
                 axis_source<3>(axis_pi<3, R>(V.qin, f8, s, sz8, P.hz), (R)(ld(V.ucur, s, 3 * f8) * iut), Sa);
-                face_flux<3, R>(t, uT_of(V.ucur, f8, s), V.qin, f8, s + sz8, sz8, [&](int n, R h) {
+                auto upper = [&](int n, R h) {
                     add_component(acc[n], n, Hz[n][tid], h, Sa, inv_dz, dt);
                     Hz[n][tid] = h;
-                });
+                };
+                if constexpr (ZRING) {
+                    // upper face: planes kk-1 .. kk+1 from the ring, kk+2 from memory (all 15
+                    // loads issued together); plane kk+2 then takes the slot kk-1 leaves
+                    double fresh[NQ];
+#pragma unroll
+                    for (int n = 0; n < NQ; ++n) fresh[n] = ld(V.qin, s + 2u * sz8, n * f8);
+                    face_flux_from<3, R>(
+                        t, uT_of(V.ucur, f8, s),
+                        [&](int n, int j) { return j < 3 ? (double)zr(kk - 1 + j, n) : fresh[n]; }, upper);
+#pragma unroll
+                    for (int n = 0; n < NQ; ++n) zr(kk + 2, n) = (R)fresh[n];
+                } else {
+                    face_flux<3, R>(t, uT_of(V.ucur, f8, s), V.qin, f8, s + sz8, sz8, upper);
+                }
             }
         } else if (cell_ok) {
             // reference work split: both faces of each direction, Newton guess from this cell
@@ -551,8 +592,10 @@ __device__ __forceinline__ void stage_tile(const StageParams &P, int tile, R (*H
 // The LDS of a stage workgroup, sized by the template arguments.
 template <bool D3, bool SHARED, typename R>
 struct StageLds {
+    static constexpr bool kRing = D3 && SHARED && sizeof(R) == 4;
     R Hs[NQ][NTHREADS];
     R Hz[D3 && SHARED ? NQ : 1][NTHREADS];
+    R Zr[kRing ? 3 : 1][kRing ? NQ : 1][NTHREADS];
 };
 
 // One event: workgroup -> tile (XCD-aware) -> stage_tile.
@@ -562,7 +605,7 @@ __global__ __launch_bounds__(NTHREADS, D3 ? VAH_BLOCKS_3D : VAH_BLOCKS_2D) void 
     const int b = (int)blockIdx.x;
     const int tile = (gridDim.x & 7) == 0 ? remap_block(b, (int)gridDim.x) : b;
     if (tile >= P.nblocks) return;  // padding workgroups: no barrier reached
-    stage_tile<D3, STAGE2, SHARED, R>(P, tile, lds.Hs, lds.Hz);
+    stage_tile<D3, STAGE2, SHARED, R>(P, tile, lds.Hs, lds.Hz, &lds.Zr[0][0][0]);
 }
 
 // Zero-gradient ghost copies of an initial state: each boundary interior cell

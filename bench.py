@@ -7,7 +7,9 @@ nucleons as Gaussians of width 0.5 fm, seed 42, peak 60 GeV/fm^3), eta plateau
 half-width 1.5 with Gaussian tails 1.4, lattice 201 x 201 x 101,
 dx = deta = 0.1, tau0 = 0.5 fm/c, dtau = 0.02 fm/c, lattice-QCD-like EOS
 tabulated at 10^4 nodes, eta/s = 0.16.  A step is one Heun-RK2 step of the
-whole lattice (two fused stage launches); the state is resident in HBM.
+whole lattice (two fused stage launches); the state is resident in HBM.  The
+default 5 warmup + 50 timed steps stay in the part of the evolution where
+every cell's Newton recovery succeeds (see the --steps help).
 
 Prints one JSON line.  --gpus above 1 is refused: this build runs on one GPU.
 """
@@ -194,16 +196,24 @@ def run_plain(V, ev, steps, tab):
 
 
 def split_errors(a, b):
-    """Relative errors of e and u^mu (relative to u^tau) of a split-precision
-    state a against the FP64 state b: max and RMS over all cells, and max over
-    the cells holding at least 1% of the event's largest e (the fluid, as
-    opposed to the dilute edge whose rounding is amplified by the Newton
-    fallbacks of the scheme)."""
+    """Errors of a split-precision state a against the FP64 state b: per-cell
+    relative error of e and of u^mu (relative to u^tau), max and RMS over all
+    cells, and the largest |delta e| relative to the event's largest e."""
     e = np.abs(a["e"] - b["e"]) / b["e"]
     u = np.max(np.abs(a["u"] - b["u"]), axis=0) / b["u"][0]
-    bulk = b["e"] >= 0.01 * b["e"].max()
     return (float(e.max()), float(np.sqrt(np.mean(e * e))), float(u.max()), float(np.sqrt(np.mean(u * u))),
-            float(e[bulk].max()), float(u[bulk].max()))
+            float(np.max(np.abs(a["e"] - b["e"])) / b["e"].max()))
+
+
+def batch_errors(V, evs, tab, steps):
+    """Split precision against FP64 for every event after the same number of
+    steps (freeze-out off): the per-event error tuples of split_errors."""
+    r = [run_batched(V, evs, tab, prec, 0.0, max_steps=[steps] * len(evs)) for prec in (V.FP32_SPLIT, V.FP64)]
+    out = np.array([split_errors(r[0]["B"].state(a, ("e", "u")), r[1]["B"].state(b, ("e", "u")))
+                    for a, b in zip(r[0]["slots"], r[1]["slots"])])
+    for x in r:
+        x["B"].close()
+    return out
 
 
 def bench_c4(V, args):
@@ -225,6 +235,7 @@ def bench_c4(V, args):
                      for s32, s64 in zip(r32["slots"], r64["slots"])])
     frozen = states == V.EVENT_FROZEN
     ef = errs[frozen] if frozen.any() else errs
+    e100 = batch_errors(V, evs, tab, 100)
     r32["B"].close()
     r64["B"].close()
     # 3. the plain per-event FP64 path, one event after the other, same step counts
@@ -265,12 +276,14 @@ def bench_c4(V, args):
         "plain_fp64_per_event": {"events_per_s": n / sum(t_plain), "wall_s": sum(t_plain),
                                  "cell_updates_per_s": cell_updates / sum(t_plain)},
         "speedup_vs_plain_fp64": sum(t_plain) / r32["wall"],
-        "split_vs_fp64_error": {"events": "frozen-out events, at their freeze-out step",
-                                "e_max": float(ef[:, 0].max()), "e_rms": float(np.sqrt(np.mean(ef[:, 1] ** 2))),
-                                "u_max": float(ef[:, 2].max()), "u_rms": float(np.sqrt(np.mean(ef[:, 3] ** 2))),
-                                "e_max_cells_above_1pct_of_max_e": float(ef[:, 4].max()),
-                                "u_max_cells_above_1pct_of_max_e": float(ef[:, 5].max()),
-                                "e_max_all_events": float(errs[:, 0].max())},
+        "split_vs_fp64_error": {
+            "step_100_all_events": {"e_max": float(e100[:, 0].max()), "e_rms": float(np.sqrt(np.mean(e100[:, 1] ** 2))),
+                                    "u_max": float(e100[:, 2].max()), "u_rms": float(np.sqrt(np.mean(e100[:, 3] ** 2))),
+                                    "de_over_max_e": float(e100[:, 4].max())},
+            "at_freeze_out": {"events": int(len(ef)), "e_max": float(ef[:, 0].max()),
+                              "e_rms": float(np.sqrt(np.mean(ef[:, 1] ** 2))), "u_max": float(ef[:, 2].max()),
+                              "u_rms": float(np.sqrt(np.mean(ef[:, 3] ** 2))), "de_over_max_e": float(ef[:, 4].max())},
+            "note": "per-cell relative errors; de_over_max_e = max |delta e| / the event's max e"},
         "steps_percentiles": {str(q): float(np.percentile(nsteps, q)) for q in (0, 50, 90, 99, 100)},
         "roofline": {"bound": "hbm", "achieved": BYTES_PER_CELL_UPDATE * cell_updates / r32["wall"] / 1e9,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -392,8 +405,12 @@ def calibrate_copy(mib):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
-    ap.add_argument("--warmup", type=int, default=10)
+    # 5 + 50 steps of the 201x201x101 event cover tau = 0.5 .. 1.6 fm/c; with its
+    # 0.5 fm nucleon hot spots the scheme loses the Newton recovery at the
+    # dilute edge from step ~80 on (tau ~ 2.1 fm/c) and the event blows up
+    # near step 100, on the CPU oracle as on the GPU
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["c4", "c5"], default="c2",
                     help="c2 (default): one 201x201x101 event; c1: 81x81x1; c4: batched events; c5: serving")
     ap.add_argument("--events", type=int, default=256, help="c4 / c5: number of events / requests")

@@ -53,6 +53,24 @@ def initial_state(V, w):
     return e
 
 
+def host_cpu():
+    """CPU model and the number of cores this process may run on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        nproc = len(os.sched_getaffinity(0))
+    except AttributeError:
+        nproc = os.cpu_count() or 1
+    return {"model": model, "nproc": nproc}
+
+
 def cpu_baseline(w, e, eos_table, seconds=15.0):
     """The CPU oracle (this project's C restatement of the reference CPU step,
     OpenMP over the host cores) on the same lattice for a few steps."""
@@ -72,9 +90,16 @@ def cpu_baseline(w, e, eos_table, seconds=15.0):
         o.step(1)
         t_used += time.perf_counter() - a
         steps += 1
+    # the reference's step ends with the checkValidity sweep
+    # (FullyDiscreteKurganovTadmorScheme.cpp:791): one sweep timed alone
+    a = time.perf_counter()
+    o.validity()
+    t_val = time.perf_counter() - a
     o.close()
     out = {"value": n * steps / t_used, "unit": "cell-updates/s", "cores": cores, "kind": "port",
-           "sample": f"{steps} full-lattice steps of the same workload on the CPU oracle ({t_used:.1f} s)"}
+           "sample": f"{steps} full-lattice steps of the same workload on the CPU oracle ({t_used:.1f} s), "
+                     "without the validity sweep",
+           "with_validity_sweep": n / (t_used / steps + t_val), "host": host_cpu()}
     # one core, as the reference's own single-threaded loops run: the middle
     # 12 eta planes of the same initial state (same per-cell work), 2 steps
     if w["nz"] > 12:

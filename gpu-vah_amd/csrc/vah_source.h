@@ -308,7 +308,7 @@ __device__ __forceinline__ void regulate(double t, QA &q, double &e, double &p, 
         double am = ar[m] * ispipi;
         if (am > rho) rho = am;
     }
-    double fac = tanh(rho) / rho;
+    double fac = tanh_over_x(rho);  // tanh(rho) / rho
     if (rho < 1.e-7) fac = 1.0;
 #pragma unroll
     for (int n = PITT; n <= PINN; ++n) q[n] *= fac;

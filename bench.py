@@ -33,10 +33,10 @@ BYTES_PER_CELL_STAGE = (368, 488)  # algorithmic bytes per cell of the stage-1 /
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "stage_kernel_hbm_traffic_201x201x101_fp64.md")
 
 WORKLOADS = {
-    "c2": dict(nx=201, ny=201, nz=101, dx=0.1, dz=0.1, dt=0.02, t0=0.5, etabar=0.16, eos="table", ic="mc",
+    "mc3d": dict(nx=201, ny=201, nz=101, dx=0.1, dz=0.1, dt=0.02, t0=0.5, etabar=0.16, eos="table", ic="mc",
                name="3+1D MC-Glauber Pb+Pb b=6 fm, 201x201x101, dx=deta=0.1, tau0=0.5, dtau=0.02, "
                     "LQCD-like EOS table (1e4 nodes), eta/s=0.16"),
-    "c1": dict(nx=81, ny=81, nz=1, dx=0.1, dz=0.1, dt=0.01, t0=0.5, etabar=0.2, eos="conformal", ic="optical",
+    "optical2d": dict(nx=81, ny=81, nz=1, dx=0.1, dz=0.1, dt=0.01, t0=0.5, etabar=0.2, eos="conformal", ic="optical",
                name="2+1D optical Glauber Au+Au b=7 fm, 81x81x1, dx=0.1, tau0=0.5, dtau=0.01, conformal EOS, "
                     "eta/s=0.2"),
 }
@@ -119,7 +119,7 @@ def cpu_baseline(w, e, eos_table, seconds=15.0):
 
 
 # ---------------------------------------------------------------------------
-# Batched (C4) and serving (C5) forms
+# Batched and serving forms
 # ---------------------------------------------------------------------------
 HBARC = 0.197326938
 T_FREEZE_GEV = 0.150
@@ -147,7 +147,7 @@ def parallel_map(fn, items, workers=16):
         return list(ex.map(fn, items))
 
 
-def c4_events(V, n, seed):
+def batch_events(V, n, seed):
     """n boost-invariant events: side ~ U{101..261}, dx ~ U[0.08, 0.12],
     dtau = 0.2 dx, b ~ U[0, 12] fm, tau0 = 0.5, eta/s = 0.16."""
     rng = np.random.default_rng(seed)
@@ -241,9 +241,9 @@ def batch_errors(V, evs, tab, steps):
     return out
 
 
-def bench_c4(V, args):
+def bench_batch(V, args):
     tab = V.eos_lqcd_table(10000, 1e-8, 1e4)
-    evs = c4_events(V, args.events, seed=2024)
+    evs = batch_events(V, args.events, seed=2024)
     cells = np.array([ev["nx"] * ev["ny"] for ev in evs])
     # 1. the batched split-precision form, freeze-out at T < 150 MeV (autotuned)
     r32 = run_batched(V, evs, tab, V.FP32_SPLIT, T_FREEZE_GEV, autotune_steps=args.autotune_steps,
@@ -338,7 +338,7 @@ def serving_requests(V, n, seed):
     return reqs, base
 
 
-def bench_c5(V, args):
+def bench_serving(V, args):
     serve = importlib.import_module("gpu-vah_amd.serve")
     tab = V.eos_lqcd_table(10000, 1e-8, 1e4)
     reqs, base = serving_requests(V, args.events, seed=77)
@@ -399,7 +399,7 @@ def measured_traffic(workload):
     FETCH_SIZE / WRITE_SIZE passes (scaled by the copy-kernel factor) and
     recorded in the committed traffic summary; None where no summary exists
     for this workload."""
-    if workload != "c2" or not os.path.exists(TRAFFIC_SUMMARY):
+    if workload != "mc3d" or not os.path.exists(TRAFFIC_SUMMARY):
         return None
     out = {}
     with open(TRAFFIC_SUMMARY) as f:
@@ -430,21 +430,22 @@ def calibrate_copy(mib):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    # 5 + 50 steps of the 201x201x101 event cover tau = 0.5 .. 1.6 fm/c; with its
-    # 0.5 fm nucleon hot spots the scheme loses the Newton recovery at the
-    # dilute edge from step ~80 on (tau ~ 2.1 fm/c) and the event blows up
-    # near step 100, on the CPU oracle as on the GPU
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["c4", "c5"], default="c2",
-                    help="c2 (default): one 201x201x101 event; c1: 81x81x1; c4: batched events; c5: serving")
-    ap.add_argument("--events", type=int, default=256, help="c4 / c5: number of events / requests")
-    ap.add_argument("--autotune-steps", type=int, default=4, help="c4: steps per variant trial (0: no autotune)")
+    ap.add_argument("--steps", type=int, default=50,
+                    help="timed steps.  5 + 50 steps of the 201x201x101 event cover tau = 0.5 .. 1.6 fm/c; with "
+                         "its 0.5 fm nucleon hot spots the scheme loses the Newton recovery at the dilute edge "
+                         "from step ~80 on (tau ~ 2.1 fm/c) and the event blows up near step 100, on the CPU "
+                         "oracle as on the GPU")
+    ap.add_argument("--warmup", type=int, default=5, help="untimed steps before the timed ones")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["batch", "serving"], default="mc3d",
+                    help="mc3d (default): one 201x201x101 MC-Glauber event; optical2d: 81x81x1 optical Glauber; "
+                         "batch: 256 boost-invariant events in one batch; serving: Poisson request stream")
+    ap.add_argument("--events", type=int, default=256, help="batch / serving: number of events / requests")
+    ap.add_argument("--autotune-steps", type=int, default=4, help="batch: steps per variant trial (0: no autotune)")
     ap.add_argument("--max-steps", type=int, default=1000,
-                    help="c4: step limit per event (events of this workload freeze out in 160-700 steps; "
+                    help="batch: step limit per event (events of this workload freeze out in 160-700 steps; "
                          "one still hot at the limit has an unstable edge)")
-    ap.add_argument("--rates", default="2,4,8,16", help="c5: Poisson arrival rates [1/s] to sweep")
-    ap.add_argument("--budget-gb", type=float, default=32.0, help="c5: HBM admission budget [GiB]")
+    ap.add_argument("--rates", default="2,4,8,16", help="serving: Poisson arrival rates [1/s] to sweep")
+    ap.add_argument("--budget-gb", type=float, default=32.0, help="serving: HBM admission budget [GiB]")
     ap.add_argument("--flux-mode", choices=["shared", "per_cell"], default="shared")
     ap.add_argument("--precision", choices=["fp64", "fp32_split"], default="fp64",
                     help="fp32_split: stencil, fluxes and sources in float, Newton recovery in double")
@@ -463,11 +464,11 @@ def main():
         return
 
     V = importlib.import_module("gpu-vah_amd")
-    if args.workload == "c4":
-        bench_c4(V, args)
+    if args.workload == "batch":
+        bench_batch(V, args)
         return
-    if args.workload == "c5":
-        bench_c5(V, args)
+    if args.workload == "serving":
+        bench_serving(V, args)
         return
     w = WORKLOADS[args.workload]
     e = initial_state(V, w)

@@ -32,7 +32,7 @@ def test_c1_shape_200_steps(oracle_mod, flux_mode):
     """81x81x1, dx 0.1, tau0 0.5, dtau 0.01, 200 steps, conformal, eta/s 0.2, P_L = e/3."""
     e = gaussian_ic(81, 81)
     a, b, tau = run_pair(oracle_mod, 81, 81, 1, 200, e, pl=e / 3, flux_mode=flux_mode)
-    worst = assert_parity(a, b, tau, label="C1 shape")
+    worst = assert_parity(a, b, tau, label="81x81x1 optical Glauber")
     print("worst field", worst)
 
 
@@ -148,7 +148,7 @@ def precision_errors(a, b):
 
 
 def test_fp32_split_precision_form(oracle_mod):
-    """C1's event with seeded +-5% lognormal noise on e, run to tau = 5 fm/c
+    """The 81x81x1 optical Glauber event with seeded +-5% lognormal noise on e, run to tau = 5 fm/c
     (450 steps) with stencil, fluxes and sources in float: e and u^mu within
     1e-4 of the FP64 oracle, total energy within 1e-6."""
     V = vah()

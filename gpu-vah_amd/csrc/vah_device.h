@@ -239,25 +239,28 @@ __device__ __forceinline__ double pt_hat(double e, double p, double pl) {
 }
 
 // ---------------------------------------------------------------------------
-// Generalised minmod slope, theta = 1.1 (FluxLimiter.cpp:13-33).  minmod of
-// two values is sign*min(|x|,|y|) when the signs (with sign(0) = +1) agree,
-// else 0 -- the reference's (sign(x)+sign(y))*min/2 without the arithmetic.
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ bool same_sign(double a, double b) { return (__double2hiint(a) ^ __double2hiint(b)) >= 0; }
-__device__ __forceinline__ bool same_sign(float a, float b) { return (__float_as_int(a) ^ __float_as_int(b)) >= 0; }
-
+// Generalised minmod slope, theta = 1.1 (FluxLimiter.cpp:13-33):
 // minmod(l, minmod(c, r)) of l = theta (y - x), c = (z - x) / 2,
-// r = theta (z - y).  It is nonzero only when l and r are nonzero with one
-// sign (c then has that sign as well); it is then the smallest of the three
-// magnitudes with that sign.  In every other case -- opposite signs, or a
-// zero, whose sign the reference counts as +1 -- one of the minima is 0.
+// r = theta (z - y), where minmod of two values is sign*min(|x|,|y|) when the
+// signs (with sign(0) = +1) agree, else 0.  The result is nonzero only when
+// a = y - x and b = z - y are nonzero with one sign; z - x then has that sign
+// too, and the result is the smallest of |l|, |c|, |r| with that sign.  In
+// every other case -- opposite signs, or a zero, whose sign the reference
+// counts as +1 -- one of the minima is 0.  Here min(|l|, |r|) is formed as
+// theta * min(|a|, |b|), equal because rounding is monotonic, and "nonzero
+// with one sign" as a * b > 0.  That test differs only when a * b underflows
+// to 0; the slope is then 0 instead of a value of at most
+// 1.1 * sqrt(5e-324) = 2.4e-162 in magnitude (2.6e-23 in float).
+// half_slope returns half of it, the step from the cell to its face: halving
+// commutes with rounding, so (theta / 2) * min(|a|, |b|) and |z - x| / 4 give
+// the halves of the two candidates exactly (short of subnormal results).
+// ---------------------------------------------------------------------------
 template <typename R>
-__device__ __forceinline__ R slope(R x, R y, R z) {
-    const R l = R(1.1) * (y - x);
-    const R c = R(0.5) * (z - x);
-    const R r = R(1.1) * (z - y);
-    const R m = fmin(fmin(fabs(l), fabs(r)), fabs(c));
-    return (m > R(0) && same_sign(l, r)) ? __builtin_copysign(m, l) : R(0);
+__device__ __forceinline__ R half_slope(R x, R y, R z) {
+    constexpr R kHalfTheta = R(1.1) * R(0.5);
+    const R a = y - x, b = z - y;
+    const R m = fmin(kHalfTheta * fmin(fabs(a), fabs(b)), R(0.25) * fabs(z - x));
+    return a * b > R(0) ? __builtin_copysign(m, a) : R(0);
 }
 
 // ---------------------------------------------------------------------------

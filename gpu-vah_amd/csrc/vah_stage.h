@@ -232,8 +232,8 @@ __device__ __forceinline__ void face_flux_from(double t, double uT_guess, Load l
 #pragma unroll
     for (int n = 0; n < NN; ++n) {
         R q0 = (R)raw[n][0], q1 = (R)raw[n][1], q2 = (R)raw[n][2], q3 = (R)raw[n][3];
-        qL[n] = q1 + R(0.5) * slope(q0, q1, q2);
-        qR[n] = q2 - R(0.5) * slope(q1, q2, q3);
+        qL[n] = q1 + half_slope(q0, q1, q2);
+        qR[n] = q2 - half_slope(q1, q2, q3);
     }
     // the Newton recovery of both sides runs in double whatever R is
     FaceState FR = face_state(t, qR[TTT], qR[TTX], qR[TTY], qR[TTN], qR[PL], qR[PITT], qR[PITX], qR[PITY], qR[PITN]);
@@ -254,8 +254,8 @@ __device__ __forceinline__ void face_flux_from(double t, double uT_guess, Load l
 #pragma unroll
     for (int n = NN; n < NQ; ++n) {
         R q0 = (R)rest[n - NN][0], q1 = (R)rest[n - NN][1], q2 = (R)rest[n - NN][2], q3 = (R)rest[n - NN][3];
-        R l = q1 + R(0.5) * slope(q0, q1, q2);
-        R r = q2 - R(0.5) * slope(q1, q2, q3);
+        R l = q1 + half_slope(q0, q1, q2);
+        R r = q2 - half_slope(q1, q2, q3);
         emit(n, cR * r + cL * l);
     }
 }
@@ -278,27 +278,27 @@ __device__ __forceinline__ double uT_of(const Buf &u, uint32_t f8, uint32_t c) {
 }
 
 // d_i of the pi components entering the axis source of direction DIR
+// (all fourteen loads issued before the differences are formed)
 template <int DIR, typename R>
 __device__ __forceinline__ AxisPi<R> axis_pi(const Buf &q, uint32_t f8, uint32_t s, uint32_t stp, double h) {
-    auto d = [&](int n) { return (R)((ld(q, s + stp, n * f8) - ld(q, s - stp, n * f8)) * h); };
-    AxisPi<R> r;
-    r.dtt = d(PITT);
-    r.dtx = d(PITX);
-    r.dty = d(PITY);
-    r.dtn = d(PITN);
-    if (DIR == 1) {
-        r.dix = d(PIXX);
-        r.diy = d(PIXY);
-        r.din = d(PIXN);
-    } else if (DIR == 2) {
-        r.dix = d(PIXY);
-        r.diy = d(PIYY);
-        r.din = d(PIYN);
-    } else {
-        r.dix = d(PIXN);
-        r.diy = d(PIYN);
-        r.din = d(PINN);
+    // pi^{tau mu} (4), then pi^{DIR mu} for mu = x, y, eta
+    constexpr int F[7] = {PITT, PITX, PITY, PITN, DIR == 1 ? PIXX : DIR == 2 ? PIXY : PIXN,
+                          DIR == 1 ? PIXY : DIR == 2 ? PIYY : PIYN, DIR == 1 ? PIXN : DIR == 2 ? PIYN : PINN};
+    double up[7], dn[7];
+#pragma unroll
+    for (int i = 0; i < 7; ++i) {
+        up[i] = ld(q, s + stp, F[i] * f8);
+        dn[i] = ld(q, s - stp, F[i] * f8);
     }
+    auto d = [&](int i) { return (R)((up[i] - dn[i]) * h); };
+    AxisPi<R> r;
+    r.dtt = d(0);
+    r.dtx = d(1);
+    r.dty = d(2);
+    r.dtn = d(3);
+    r.dix = d(4);
+    r.diy = d(5);
+    r.din = d(6);
     return r;
 }
 
@@ -366,8 +366,12 @@ __device__ __forceinline__ double finish_cell(const StageParams &P, const Views 
                                               double (&acc)[NQ], bool &nan, bool &fallback) {
     const uint32_t f8 = V.f8;
     if (STAGE2) {
+        // all fifteen loads first, so that they are in flight together
+        double qn[NQ];
 #pragma unroll
-        for (int n = 0; n < NQ; ++n) acc[n] = 0.5 * (acc[n] + ld(V.qn, s, n * f8));
+        for (int n = 0; n < NQ; ++n) qn[n] = ld(V.qn, s, n * f8);
+#pragma unroll
+        for (int n = 0; n < NQ; ++n) acc[n] = 0.5 * (acc[n] + qn[n]);
     }
     Prim r = recover(P.eos, P.t_rec, acc, ld(V.e_in, s, 0), uT_of(V.uguess, f8, s));
     nan = nan || prim_nan(r);

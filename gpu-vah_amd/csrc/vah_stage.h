@@ -217,8 +217,8 @@ __device__ __forceinline__ Views make_views(const StageParams &P) {
 // (HalfSiteExtrapolation.cpp:12-26); emit(n, H_n) receives the 15 components
 // in order.  The nine components the Newton solve needs stay in registers,
 // the other six are reconstructed after it, which keeps the live state small.
-template <int DIR, typename R, typename Load, typename Emit>
-__device__ __forceinline__ void face_flux_from(double t, double uT_guess, Load load, Emit emit) {
+template <int DIR, typename R, typename Guess, typename Load, typename Emit>
+__device__ __forceinline__ void face_flux_from(double t, Guess guess, Load load, Emit emit) {
     constexpr int NN = 9;  // TTT..PITN
     // all loads of the nine fields first, so that they are in flight together;
     // load(n, j) gives field n at stencil position j (0..3: c-2st .. c+st)
@@ -228,6 +228,9 @@ __device__ __forceinline__ void face_flux_from(double t, double uT_guess, Load l
 #pragma unroll
         for (int j = 0; j < 4; ++j) raw[n][j] = load(n, j);
     }
+    // the Newton guess |u_T| (loads of its own) after the stencil loads, so
+    // that one wait covers both
+    const double uT_guess = guess();
     R qL[NN], qR[NN];
 #pragma unroll
     for (int n = 0; n < NN; ++n) {
@@ -265,11 +268,11 @@ __device__ __forceinline__ void face_flux_from(double t, double uT_guess, Load l
 // (HalfSiteExtrapolation.cpp:12-26); emit(n, H_n) receives the 15 components
 // in order.  The nine components the Newton solve needs stay in registers,
 // the other six are loaded and reconstructed after it.
-template <int DIR, typename R, typename Emit>
-__device__ __forceinline__ void face_flux(double t, double uT_guess, const Buf &q, uint32_t f8, uint32_t c,
+template <int DIR, typename R, typename Guess, typename Emit>
+__device__ __forceinline__ void face_flux(double t, Guess guess, const Buf &q, uint32_t f8, uint32_t c,
                                           uint32_t stp, Emit emit) {
     face_flux_from<DIR, R>(
-        t, uT_guess, [&](int n, int j) { return ld(q, c + (uint32_t)j * stp - 2u * stp, n * f8); }, emit);
+        t, guess, [&](int n, int j) { return ld(q, c + (uint32_t)j * stp - 2u * stp, n * f8); }, emit);
 }
 
 __device__ __forceinline__ double uT_of(const Buf &u, uint32_t f8, uint32_t c) {
@@ -290,6 +293,7 @@ __device__ __forceinline__ AxisPi<R> axis_pi(const Buf &q, uint32_t f8, uint32_t
         up[i] = ld(q, s + stp, F[i] * f8);
         dn[i] = ld(q, s - stp, F[i] * f8);
     }
+    __builtin_amdgcn_sched_barrier(0);  // keep the loads together (not sunk to their uses)
     auto d = [&](int i) { return (R)((up[i] - dn[i]) * h); };
     AxisPi<R> r;
     r.dtt = d(0);
@@ -490,7 +494,7 @@ __device__ __forceinline__ void stage_tile(const StageParams &P, int tile, R (*H
                 const int gi = i0 + fi, gj = j0 + fj;  // interior index of the right cell
                 if (fj < TY && gi <= P.nx && gj < P.ny) {
                     const uint32_t c = (plane + (uint32_t)(gj + 2) * sy + (uint32_t)(gi + 2)) * 8u;
-                    face_flux<1, R>(t, uT_of(V.ucur, f8, c - 8), V.qin, f8, c, 8,
+                    face_flux<1, R>(t, [&] { return uT_of(V.ucur, f8, c - 8); }, V.qin, f8, c, 8,
                                     [&](int n, R h) { Hs[n][tid] = h; });
                 }
             }
@@ -509,7 +513,7 @@ __device__ __forceinline__ void stage_tile(const StageParams &P, int tile, R (*H
                 const int gi = i0 + fi, gj = j0 + fj;
                 if (fj <= TY && gi < P.nx && gj <= P.ny) {
                     const uint32_t c = (plane + (uint32_t)(gj + 2) * sy + (uint32_t)(gi + 2)) * 8u;
-                    face_flux<2, R>(t, uT_of(V.ucur, f8, c - sy8), V.qin, f8, c, sy8,
+                    face_flux<2, R>(t, [&] { return uT_of(V.ucur, f8, c - sy8); }, V.qin, f8, c, sy8,
                                     [&](int n, R h) { Hs[n][tid] = h; });
                 }
             }
@@ -526,17 +530,23 @@ __device__ __forceinline__ void stage_tile(const StageParams &P, int tile, R (*H
             if (D3 && cell_ok) {
                 if (kk == k0) {
                     if constexpr (ZRING) {
-                        // lower face: planes k0-2 .. k0+1 from memory, the ring keeps k0-1 .. k0+1
-                        face_flux_from<3, R>(
-                            t, uT_of(V.ucur, f8, s - sz8),
+                        // lower face: planes k0-1 .. k0+1 into the ring first (fifteen
+                        // loads in flight at a time), plane k0-2 from memory
+#pragma unroll
+                        for (int j = 1; j < 4; ++j) {
+                            double v[NQ];
+#pragma unroll
+                            for (int n = 0; n < NQ; ++n) v[n] = ld(V.qin, s + (uint32_t)j * sz8 - 2u * sz8, n * f8);
+#pragma unroll
+                            for (int n = 0; n < NQ; ++n) zr(kk + j - 2, n) = (R)v[n];
+                        }
+                        face_flux_from<3, R>(t, [&] { return uT_of(V.ucur, f8, s - sz8); },
                             [&](int n, int j) {
-                                const double v = ld(V.qin, s + (uint32_t)j * sz8 - 2u * sz8, n * f8);
-                                if (j >= 1) zr(kk + j - 2, n) = (R)v;
-                                return v;
+                                return j == 0 ? ld(V.qin, s - 2u * sz8, n * f8) : (double)zr(kk + j - 2, n);
                             },
                             [&](int n, R h) { Hz[n][tid] = h; });
                     } else {
-                        face_flux<3, R>(t, uT_of(V.ucur, f8, s - sz8), V.qin, f8, s, sz8,
+                        face_flux<3, R>(t, [&] { return uT_of(V.ucur, f8, s - sz8); }, V.qin, f8, s, sz8,
                                         [&](int n, R h) { Hz[n][tid] = h; });
                     }
                 }
@@ -552,13 +562,12 @@ __device__ __forceinline__ void stage_tile(const StageParams &P, int tile, R (*H
                     double fresh[NQ];
 #pragma unroll
                     for (int n = 0; n < NQ; ++n) fresh[n] = ld(V.qin, s + 2u * sz8, n * f8);
-                    face_flux_from<3, R>(
-                        t, uT_of(V.ucur, f8, s),
+                    face_flux_from<3, R>(t, [&] { return uT_of(V.ucur, f8, s); },
                         [&](int n, int j) { return j < 3 ? (double)zr(kk - 1 + j, n) : fresh[n]; }, upper);
 #pragma unroll
                     for (int n = 0; n < NQ; ++n) zr(kk + 2, n) = (R)fresh[n];
                 } else {
-                    face_flux<3, R>(t, uT_of(V.ucur, f8, s), V.qin, f8, s + sz8, sz8, upper);
+                    face_flux<3, R>(t, [&] { return uT_of(V.ucur, f8, s); }, V.qin, f8, s + sz8, sz8, upper);
                 }
             }
         } else if (cell_ok) {
@@ -567,22 +576,22 @@ __device__ __forceinline__ void stage_tile(const StageParams &P, int tile, R (*H
             {
                 R Sa[4];
                 axis_source<1>(axis_pi<1, R>(V.qin, f8, s, 8, P.hx), (R)(ld(V.ucur, s, f8) * iut), Sa);
-                face_flux<1, R>(t, uTs, V.qin, f8, s + 8, 8, [&](int n, R h) { Hs[n][tid] = h; });
-                face_flux<1, R>(t, uTs, V.qin, f8, s, 8,
+                face_flux<1, R>(t, [&] { return uTs; }, V.qin, f8, s + 8, 8, [&](int n, R h) { Hs[n][tid] = h; });
+                face_flux<1, R>(t, [&] { return uTs; }, V.qin, f8, s, 8,
                                 [&](int n, R h) { add_component(acc[n], n, h, Hs[n][tid], Sa, inv_dx, dt); });
             }
             {
                 R Sa[4];
                 axis_source<2>(axis_pi<2, R>(V.qin, f8, s, sy8, P.hy), (R)(ld(V.ucur, s, 2 * f8) * iut), Sa);
-                face_flux<2, R>(t, uTs, V.qin, f8, s + sy8, sy8, [&](int n, R h) { Hs[n][tid] = h; });
-                face_flux<2, R>(t, uTs, V.qin, f8, s, sy8,
+                face_flux<2, R>(t, [&] { return uTs; }, V.qin, f8, s + sy8, sy8, [&](int n, R h) { Hs[n][tid] = h; });
+                face_flux<2, R>(t, [&] { return uTs; }, V.qin, f8, s, sy8,
                                 [&](int n, R h) { add_component(acc[n], n, h, Hs[n][tid], Sa, inv_dy, dt); });
             }
             if (D3) {
                 R Sa[4];
                 axis_source<3>(axis_pi<3, R>(V.qin, f8, s, sz8, P.hz), (R)(ld(V.ucur, s, 3 * f8) * iut), Sa);
-                face_flux<3, R>(t, uTs, V.qin, f8, s + sz8, sz8, [&](int n, R h) { Hs[n][tid] = h; });
-                face_flux<3, R>(t, uTs, V.qin, f8, s, sz8,
+                face_flux<3, R>(t, [&] { return uTs; }, V.qin, f8, s + sz8, sz8, [&](int n, R h) { Hs[n][tid] = h; });
+                face_flux<3, R>(t, [&] { return uTs; }, V.qin, f8, s, sz8,
                                 [&](int n, R h) { add_component(acc[n], n, h, Hs[n][tid], Sa, inv_dz, dt); });
             }
         }

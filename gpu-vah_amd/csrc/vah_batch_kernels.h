@@ -88,13 +88,24 @@ __global__ __launch_bounds__(NTHREADS, BLOCKS) void batch_stage_kernel(const Bat
     const int n = ntiles[D3 ? 1 : 0];
     const int xcd = (int)(blockIdx.x & 7u), loc = (int)(blockIdx.x >> 3), nloc = (int)(gridDim.x >> 3);
     const int lo = (int)(((long long)n * xcd) >> 3), hi = (int)(((long long)n * (xcd + 1)) >> 3);
-    for (int w = lo + loc; w < hi; w += nloc) {
-        const int2 tr = tiles[w];
+    int w = lo + loc;
+    if (w >= hi) return;
+    int2 tr = tiles[w];
+    for (;;) {
+        // the next list entry is fetched while this tile runs
+        const int wn = w + nloc;
+        const int2 next = wn < hi ? tiles[wn] : tr;
         const int slot = __builtin_amdgcn_readfirstlane(tr.x);
         const int tile = __builtin_amdgcn_readfirstlane(tr.y);
         BatchDyn *d = dyn + slot;
-        if (__builtin_amdgcn_readfirstlane(d->live) == 0) continue;
+        const int live = __builtin_amdgcn_readfirstlane(d->live);
         const long long nstep = d->nstep;
+        if (live == 0) {
+            if (wn >= hi) break;
+            w = wn;
+            tr = next;
+            continue;
+        }
         const BatchEvent &E = ev[slot];
         StageParams P = E.base;
         batch_roles(E, STAGE2, nstep, P);
@@ -105,6 +116,9 @@ __global__ __launch_bounds__(NTHREADS, BLOCKS) void batch_stage_kernel(const Bat
         P.emax = STAGE2 ? &d->emax : nullptr;
         stage_tile<D3, STAGE2, true, R, TX, TY>(P, tile, lds.Hs, lds.Hz, &lds.Zr[0][0][0]);
         __syncthreads();  // the next tile rewrites the face buffers
+        if (wn >= hi) break;
+        w = wn;
+        tr = next;
     }
 }
 

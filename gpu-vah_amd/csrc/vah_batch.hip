@@ -91,6 +91,7 @@ struct vah_batch {
     std::vector<HostEvent> events;
     BatchEvent *ev_dev;
     BatchDyn *dyn_dev;
+    BatchStageRec *rec_dev;  // per slot and stage, written at the start of each step
     int *active_dev;
     int *counts_dev;
     int2 *tiles_dev;   // tiles2 at [0, cap2), tiles3 at [cap2, cap2 + cap3)
@@ -262,9 +263,11 @@ int grid_of(const vah_batch *b) {
 
 // enqueue one step of every live event
 void enqueue_step(vah_batch *b, bool has2, bool has3) {
+    const int blocks = (b->max_events + 63) / 64;
+    hipLaunchKernelGGL(batch_prepare_kernel, dim3(blocks), dim3(64), 0, b->stream, b->ev_dev, b->dyn_dev,
+                       b->active_dev, b->counts_dev + 2, b->rec_dev);
     BatchLaunch L;
-    L.ev = b->ev_dev;
-    L.dyn = b->dyn_dev;
+    L.rec = b->rec_dev;
     L.tiles2 = b->tiles_dev;
     L.tiles3 = b->tiles_dev + b->cap2;
     L.counts = b->counts_dev;
@@ -276,7 +279,6 @@ void enqueue_step(vah_batch *b, bool has2, bool has3) {
         if (has2) fn(L, false, stage2 != 0, fp32);
         if (has3) fn(L, true, stage2 != 0, fp32);
     }
-    const int blocks = (b->max_events + 63) / 64;
     hipLaunchKernelGGL(batch_advance_kernel, dim3(blocks), dim3(64), 0, b->stream, b->ev_dev, b->dyn_dev,
                        b->active_dev, b->counts_dev + 2);
 }
@@ -420,6 +422,7 @@ void vah_batch_destroy(vah_batch *b) {
     (void)hipFree(b->arena);
     (void)hipFree(b->ev_dev);
     (void)hipFree(b->dyn_dev);
+    (void)hipFree(b->rec_dev);
     (void)hipFree(b->active_dev);
     (void)hipFree(b->counts_dev);
     (void)hipFree(b->tiles_dev);
@@ -490,11 +493,13 @@ int vah_batch_create(const vah_batch_options *opt_in, vah_batch **out) {
     const size_t ne = (size_t)opt.max_events;
     if (hipMalloc(&b->ev_dev, sizeof(BatchEvent) * ne) != hipSuccess ||
         hipMalloc(&b->dyn_dev, sizeof(BatchDyn) * ne) != hipSuccess ||
+        hipMalloc(&b->rec_dev, sizeof(BatchStageRec) * 2 * ne) != hipSuccess ||
         hipMalloc(&b->active_dev, sizeof(int) * ne) != hipSuccess ||
         hipMalloc(&b->counts_dev, sizeof(int) * 4) != hipSuccess ||
         hipHostMalloc(&b->dyn_host, sizeof(BatchDyn) * ne) != hipSuccess)
         return cleanup(fail(VAH_ERR_NOMEM, "vah_batch_create: event tables"));
     if (hipMemsetAsync(b->dyn_dev, 0, sizeof(BatchDyn) * ne, b->stream) != hipSuccess ||
+        hipMemsetAsync(b->rec_dev, 0, sizeof(BatchStageRec) * 2 * ne, b->stream) != hipSuccess ||
         hipMemsetAsync(b->counts_dev, 0, sizeof(int) * 4, b->stream) != hipSuccess)
         return cleanup(fail(VAH_ERR_HIP, "vah_batch_create: table clear"));
     b->cap2 = b->cap3 = 4096;

@@ -77,12 +77,46 @@ __device__ __forceinline__ double batch_tau(const BatchEvent &E, long long n) {
     return __dadd_rn(E.t0, __dmul_rn((double)n, E.dt));
 }
 
+// The resolved parameters of one event for one stage of its current step,
+// written by batch_prepare_kernel at the start of the step, so that a tile
+// needs one batch of uniform loads before its work starts.
+struct BatchStageRec {
+    StageParams P;
+    int live;
+    int pad_;
+};
+
+// Start of a step: one lane per resident event resolves the buffer roles and
+// times of both stages from its step counter.  active[0 .. *nactive) lists
+// resident slots; rec[2 slot + stage].
+static __global__ void batch_prepare_kernel(const BatchEvent *__restrict__ ev, BatchDyn *dyn,
+                                            const int *__restrict__ active, const int *__restrict__ nactive,
+                                            BatchStageRec *rec) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= *nactive) return;
+    const int slot = active[i];
+    BatchDyn *d = dyn + slot;
+    const BatchEvent &E = ev[slot];
+    const long long nstep = d->nstep;
+    const double t = batch_tau(E, nstep);
+    for (int stage2 = 0; stage2 < 2; ++stage2) {
+        BatchStageRec &r = rec[2 * slot + stage2];
+        r.P = E.base;
+        batch_roles(E, stage2 != 0, nstep, r.P);
+        r.P.t_src = stage2 ? t + E.dt : t;
+        r.P.t_rec = t + E.dt;
+        r.P.counters = d->counters;
+        r.P.emax = stage2 ? &d->emax : nullptr;
+        r.live = d->live;
+    }
+}
+
 // One stage over the tile list of one dimensionality.  tiles[w] = (slot,
 // tile index within the event); ntiles[0] / ntiles[1] count the 2+1D / 3+1D
 // lists.  gridDim.x is a multiple of 8 (one share per XCD).
 template <bool D3, bool STAGE2, typename R, int TX, int TY, int BLOCKS>
-__global__ __launch_bounds__(NTHREADS, BLOCKS) void batch_stage_kernel(const BatchEvent *__restrict__ ev,
-                                                                      BatchDyn *dyn, const int2 *__restrict__ tiles,
+__global__ __launch_bounds__(NTHREADS, BLOCKS) void batch_stage_kernel(const BatchStageRec *__restrict__ rec,
+                                                                      const int2 *__restrict__ tiles,
                                                                       const int *__restrict__ ntiles) {
     __shared__ StageLds<D3, true, R> lds;
     const int n = ntiles[D3 ? 1 : 0];
@@ -97,25 +131,11 @@ __global__ __launch_bounds__(NTHREADS, BLOCKS) void batch_stage_kernel(const Bat
         const int2 next = wn < hi ? tiles[wn] : tr;
         const int slot = __builtin_amdgcn_readfirstlane(tr.x);
         const int tile = __builtin_amdgcn_readfirstlane(tr.y);
-        BatchDyn *d = dyn + slot;
-        const int live = __builtin_amdgcn_readfirstlane(d->live);
-        const long long nstep = d->nstep;
-        if (live == 0) {
-            if (wn >= hi) break;
-            w = wn;
-            tr = next;
-            continue;
+        const BatchStageRec &r = rec[2 * slot + (STAGE2 ? 1 : 0)];
+        if (r.live != 0) {
+            stage_tile<D3, STAGE2, true, R, TX, TY>(r.P, tile, lds.Hs, lds.Hz, &lds.Zr[0][0][0]);
+            __syncthreads();  // the next tile rewrites the face buffers
         }
-        const BatchEvent &E = ev[slot];
-        StageParams P = E.base;
-        batch_roles(E, STAGE2, nstep, P);
-        const double t = batch_tau(E, nstep);
-        P.t_src = STAGE2 ? t + E.dt : t;
-        P.t_rec = t + E.dt;
-        P.counters = d->counters;
-        P.emax = STAGE2 ? &d->emax : nullptr;
-        stage_tile<D3, STAGE2, true, R, TX, TY>(P, tile, lds.Hs, lds.Hz, &lds.Zr[0][0][0]);
-        __syncthreads();  // the next tile rewrites the face buffers
         if (wn >= hi) break;
         w = wn;
         tr = next;
@@ -155,8 +175,7 @@ static __global__ void batch_advance_kernel(const BatchEvent *__restrict__ ev, B
 // Host-side launch of one variant's kernels (one translation unit per
 // variant instantiates them).
 struct BatchLaunch {
-    const BatchEvent *ev;
-    BatchDyn *dyn;
+    const BatchStageRec *rec;
     const int2 *tiles2, *tiles3;  // 2+1D and 3+1D tile lists
     const int *counts;            // [0] 2+1D tiles, [1] 3+1D tiles, [2] active events
     int grid;                     // persistent workgroups (multiple of 8)

@@ -21,7 +21,7 @@ template <bool D3, bool S2, typename R>
 void launch_one(const BatchLaunch &L) {
     constexpr int B = D3 ? kBlocks3D : VAH_BV_B;
     hipLaunchKernelGGL((batch_stage_kernel<D3, S2, R, VAH_BV_TX, VAH_BV_TY, B>), dim3(L.grid), dim3(NTHREADS),
-                       0, L.stream, L.ev, L.dyn, D3 ? L.tiles3 : L.tiles2, L.counts);
+                       0, L.stream, L.rec, D3 ? L.tiles3 : L.tiles2, L.counts);
 }
 
 template <typename R>

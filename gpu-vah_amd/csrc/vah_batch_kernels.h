@@ -51,7 +51,6 @@ __device__ __forceinline__ void batch_roles(const BatchEvent &E, bool stage2, lo
         P.qn = E.q[a];
         P.ucur = E.u[uc];
         P.uprev = E.u[up];
-        P.uguess = E.u[uc];
         P.uout = E.u[2];
         P.e_in = E.e[0];
         P.p_in = E.p[0];
@@ -63,7 +62,6 @@ __device__ __forceinline__ void batch_roles(const BatchEvent &E, bool stage2, lo
         P.qn = E.q[a];
         P.ucur = E.u[2];
         P.uprev = E.u[uc];
-        P.uguess = E.u[uc];
         P.uout = E.u[up];
         P.e_in = E.e[1];
         P.p_in = E.p[1];

@@ -217,7 +217,6 @@ StageParams make_params(const vah_ctx *c, int stage, double t) {
         P.qn = const_q[c->qi];
         P.ucur = c->useq[c->uc];
         P.uprev = c->useq[c->up];
-        P.uguess = c->useq[c->uc];
         P.uout = c->useq[c->us];
         P.t_src = t;
         P.t_rec = t + c->lat.dt;
@@ -227,7 +226,6 @@ StageParams make_params(const vah_ctx *c, int stage, double t) {
         P.qn = const_q[c->qi];
         P.ucur = c->useq[c->us];
         P.uprev = c->useq[c->uc];
-        P.uguess = c->useq[c->uc];
         P.uout = c->useq[c->up];
         P.t_src = t + c->lat.dt;
         P.t_rec = t + c->lat.dt;

@@ -53,7 +53,6 @@ struct StageParams {
     double *e_out, *p_out;
     const double *ucur;   // u used for gradients and face Newton guesses
     const double *uprev;  // u one level back (time derivative)
-    const double *uguess; // u whose |u_T| seeds the cell recovery
     double *uout;
     long fs;              // elements between fields of a slab
     int nx, ny, nz;       // interior cells
@@ -188,7 +187,7 @@ __device__ __forceinline__ void st(const Buf &b, uint32_t cell, uint32_t field, 
 
 // the buffers of one stage, and the byte strides of the layout
 struct Views {
-    Buf qin, qout, qn, e_in, p_in, e_out, p_out, ucur, uprev, uguess, uout;
+    Buf qin, qout, qn, e_in, p_in, e_out, p_out, ucur, uprev, uout;
     uint32_t f8;        // field to field
     uint32_t sy8, sz8;  // row to row, plane to plane
 };
@@ -204,7 +203,6 @@ __device__ __forceinline__ Views make_views(const StageParams &P) {
     v.p_out = make_buf(P.p_out, 1, P.fs);
     v.ucur = make_buf(P.ucur, 4, P.fs);
     v.uprev = make_buf(P.uprev, 4, P.fs);
-    v.uguess = make_buf(P.uguess, 4, P.fs);
     v.uout = make_buf(P.uout, 4, P.fs);
     v.f8 = (uint32_t)(P.fs * 8);
     v.sy8 = (uint32_t)(P.sy * 8);
@@ -364,10 +362,16 @@ __device__ __forceinline__ void load_src_in(const StageParams &P, const Views &V
     }
 }
 
+// Inputs of a cell's recovery besides its conserved variables: the e its
+// fallback scales and the |u_T| its Newton solve starts from.
+struct Recovery {
+    double e_prev, uT_guess;
+};
+
 // Stage-2 convex combination, recovery, regulation, store with ghost copies.
 template <bool D3, bool STAGE2>
 __device__ __forceinline__ double finish_cell(const StageParams &P, const Views &V, uint32_t s, int ii, int jj, int kk,
-                                              double (&acc)[NQ], bool &nan, bool &fallback) {
+                                              double (&acc)[NQ], const Recovery &rec, bool &nan, bool &fallback) {
     const uint32_t f8 = V.f8;
     if (STAGE2) {
         // all fifteen loads first, so that they are in flight together
@@ -377,7 +381,7 @@ __device__ __forceinline__ double finish_cell(const StageParams &P, const Views 
 #pragma unroll
         for (int n = 0; n < NQ; ++n) acc[n] = 0.5 * (acc[n] + qn[n]);
     }
-    Prim r = recover(P.eos, P.t_rec, acc, ld(V.e_in, s, 0), uT_of(V.uguess, f8, s));
+    Prim r = recover(P.eos, P.t_rec, acc, rec.e_prev, rec.uT_guess);
     nan = nan || prim_nan(r);
     fallback = !r.ok;
     double e = r.e, p = r.p;
@@ -472,6 +476,8 @@ __device__ __forceinline__ void stage_tile(const StageParams &P, int tile, R (*H
         const uint32_t s = (plane + (uint32_t)(jj + 2) * sy + (uint32_t)(ii + 2)) * 8u;
         bool nan = false, fallback = false;
         double acc[NQ];
+        Recovery rec{0.0, 0.0};
+        double iut = 0.0;
 
         // ---- sources: acc = q + dt S (eulerStepKernelSource)
         if (cell_ok) {
@@ -484,8 +490,15 @@ __device__ __forceinline__ void stage_tile(const StageParams &P, int tile, R (*H
             source_terms<R>(P.eos, in, Q, S, nan);
 #pragma unroll
             for (int n = 0; n < NQ; ++n) acc[n] = Q[n] + dt * (double)S[n];
+            // the cell recovery's inputs come from the same loads: e at the
+            // stage's input level (its fallback base) and |u_T| of the velocity
+            // that seeds it, u_cur in stage 1 and u_prev in stage 2
+            // (FullyDiscreteKurganovTadmorScheme.cpp:761-780)
+            rec.e_prev = in.e;
+            rec.uT_guess = STAGE2 ? __builtin_sqrt(fabs(in.uxp * in.uxp + in.uyp * in.uyp))
+                                  : __builtin_sqrt(fabs(in.ux * in.ux + in.uy * in.uy));
+            iut = rcp(in.ut);
         }
-        const double iut = cell_ok ? rcp(ld(V.ucur, s, 0)) : 0.0;
 
         if (SHARED) {
             // ---- x faces: (TX+1) x TY, face f between cells (fi-1, fj) and (fi, fj)
@@ -597,7 +610,7 @@ __device__ __forceinline__ void stage_tile(const StageParams &P, int tile, R (*H
         }
 
         double e_new = 0.0;
-        if (cell_ok) e_new = finish_cell<D3, STAGE2>(P, V, s, ii, jj, kk, acc, nan, fallback);
+        if (cell_ok) e_new = finish_cell<D3, STAGE2>(P, V, s, ii, jj, kk, acc, rec, nan, fallback);
         count_flags(P, nan, fallback, e_new);
     }
 }

@@ -629,10 +629,10 @@ template <bool D3, bool STAGE2, bool SHARED, typename R>
 __global__ __launch_bounds__(NTHREADS, D3 ? VAH_BLOCKS_3D : VAH_BLOCKS_2D) void stage_kernel(StageParams P_arg) {
     // The parameters are read where they are used, through the kernel-argument
     // segment (P_arg is its first and only argument, at offset 0), instead of
-    // from the argument copy the compiler keeps in scalar registers: with some
-    // ninety uniform values live across the plane loop those registers spilled
-    // (114 of them, plus 28 vector registers); read in place the kernel spills
-    // 38 and 16 and runs 6 % faster.
+    // from the argument copy the compiler otherwise keeps in scalar registers
+    // across the plane loop: some ninety uniform values there made it spill
+    // about three times as many scalar registers and twice as many vector
+    // registers as it does reading them in place.
     typedef __attribute__((address_space(4))) const StageParams KernargParams;
     const StageParams &P = *(const StageParams *)(KernargParams *)__builtin_amdgcn_kernarg_segment_ptr();
     (void)P_arg;

@@ -19,6 +19,7 @@
 #include <stdint.h>
 
 #include "vah_eos_wb.h"
+#include "vah_log.h"
 
 namespace vah {
 
@@ -115,7 +116,7 @@ __device__ __forceinline__ double conformal_T(double e, const uint32_t *__restri
 }
 
 __device__ __forceinline__ void table_lookup(const Eos &E, double e, double *p, double *T) {
-    double f = (log(e) - E.lne0) * E.inv_h;
+    double f = (eos_log(e) - E.lne0) * E.inv_h;
     int last = E.n - 1;
     if (!(f >= 0.0)) {
         double r = e / E.e_first;

@@ -20,6 +20,7 @@
 
 #include "vah_eos_wb.h"
 #include "vah_log.h"
+#include "vah_tanh.h"
 
 namespace vah {
 
@@ -53,16 +54,6 @@ __device__ __forceinline__ double fdiv(double a, double b) {
     double q = a * r;
     double e = __builtin_fma(-b, q, a);
     return __builtin_fma(e, r, q);
-}
-
-// tanh(x) / x for x > 0, +inf and NaN, through E = e^{2x} - 1:
-// tanh x = E / (E + 2), which keeps full relative accuracy for small x (no
-// cancellation) and needs a third of the registers and half the instructions
-// of the library tanh.  Above x = 20, tanh x rounds to 1 in double.
-__device__ __forceinline__ double tanh_over_x(double x) {
-    if (x > 20.0) return 1.0 / x;
-    const double E = expm1(2.0 * x);
-    return fdiv(E, (E + 2.0) * x);
 }
 
 __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }

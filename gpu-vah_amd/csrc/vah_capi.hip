@@ -494,7 +494,7 @@ int vah_create(const vah_lattice *lat, const vah_hydro *hydro, const vah_options
     // eta planes per workgroup: 4 in double precision; 8 in the split form, whose
     // eta-face plane ring amortises its fill over a longer march (both measured
     // fastest at 201x201x101)
-    const int kz_default = opt.precision == VAH_FP32_SPLIT ? 6 : 3;
+    const int kz_default = 3;
     c->kz = lat->nz > 1 ? std::min(opt.eta_chunk > 0 ? opt.eta_chunk : kz_default, lat->nz) : 1;
     c->zchunks = (lat->nz + c->kz - 1) / c->kz;
     c->nblocks = c->tiles_x * c->tiles_y * c->zchunks;

@@ -88,7 +88,7 @@ typedef struct vah_options {
     int flux_mode;             /* enum vah_flux_mode */
     int device;                /* HIP device ordinal */
     void *stream;              /* hipStream_t to launch on; NULL: a library-owned stream */
-    int eta_chunk;             /* eta planes one workgroup marches through (3+1D); 0: 3 (FP64), 6 (split) */
+    int eta_chunk;             /* eta planes one workgroup marches through (3+1D); 0: 3 */
 } vah_options;
 
 typedef struct vah_stats {

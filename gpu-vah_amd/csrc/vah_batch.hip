@@ -35,7 +35,7 @@ namespace vah {
 // one launch function per compiled variant (vah_batch_variant.hip)
 void batch_stage_v16x15b2(const BatchLaunch &, bool, bool, bool);
 void batch_stage_v32x7b2(const BatchLaunch &, bool, bool, bool);
-void batch_stage_v16x15b1(const BatchLaunch &, bool, bool, bool);
+void batch_stage_v32x7b3(const BatchLaunch &, bool, bool, bool);
 void batch_stage_v16x15b3(const BatchLaunch &, bool, bool, bool);
 }  // namespace vah
 
@@ -51,12 +51,12 @@ struct Variant {
 };
 
 const Variant kVariants[] = {
-    {{16, 15, 2, 8}, batch_stage_v16x15b2},
-    {{16, 15, 2, 4}, batch_stage_v16x15b2},
-    {{32, 7, 2, 8}, batch_stage_v32x7b2},
-    {{32, 7, 2, 4}, batch_stage_v32x7b2},
-    {{16, 15, 1, 8}, batch_stage_v16x15b1},
-    {{16, 15, 3, 8}, batch_stage_v16x15b3},
+    {{16, 15, 2, 3}, batch_stage_v16x15b2},
+    {{16, 15, 2, 6}, batch_stage_v16x15b2},
+    {{32, 7, 2, 3}, batch_stage_v32x7b2},
+    {{32, 7, 2, 6}, batch_stage_v32x7b2},
+    {{16, 15, 3, 3}, batch_stage_v16x15b3},
+    {{32, 7, 3, 3}, batch_stage_v32x7b3},
 };
 constexpr int kNumVariants = (int)(sizeof kVariants / sizeof kVariants[0]);
 
@@ -255,9 +255,11 @@ int upload_lists(vah_batch *b) {
     return VAH_OK;
 }
 
-int grid_of(const vah_batch *b) {
+// persistent workgroups: the variant's occupancy, at most 2 per CU for 3+1D
+// tiles (their face and eta buffers fill half a CU's LDS)
+int grid_of(const vah_batch *b, bool d3) {
     const vah_batch_variant &v = kVariants[b->variant].info;
-    int g = b->num_cus * v.blocks_per_cu;
+    int g = b->num_cus * (d3 ? std::min(v.blocks_per_cu, 2) : v.blocks_per_cu);
     return std::max(8, g / 8 * 8);
 }
 
@@ -271,7 +273,8 @@ void enqueue_step(vah_batch *b, bool has2, bool has3) {
     L.tiles2 = b->tiles_dev;
     L.tiles3 = b->tiles_dev + b->cap2;
     L.counts = b->counts_dev;
-    L.grid = grid_of(b);
+    L.grid = grid_of(b, false);
+    L.grid3 = grid_of(b, true);
     L.stream = b->stream;
     const bool fp32 = b->opt.precision == VAH_FP32_SPLIT;
     BatchStageFn fn = kVariants[b->variant].fn;

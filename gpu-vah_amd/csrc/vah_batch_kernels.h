@@ -176,7 +176,7 @@ struct BatchLaunch {
     const BatchStageRec *rec;
     const int2 *tiles2, *tiles3;  // 2+1D and 3+1D tile lists
     const int *counts;            // [0] 2+1D tiles, [1] 3+1D tiles, [2] active events
-    int grid;                     // persistent workgroups (multiple of 8)
+    int grid, grid3;              // persistent workgroups for 2+1D / 3+1D tiles (multiples of 8)
     hipStream_t stream;
 };
 

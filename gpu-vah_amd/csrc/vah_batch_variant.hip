@@ -20,7 +20,7 @@ constexpr int kBlocks3D = VAH_BV_B > 2 ? 2 : VAH_BV_B;
 template <bool D3, bool S2, typename R>
 void launch_one(const BatchLaunch &L) {
     constexpr int B = D3 ? kBlocks3D : VAH_BV_B;
-    hipLaunchKernelGGL((batch_stage_kernel<D3, S2, R, VAH_BV_TX, VAH_BV_TY, B>), dim3(L.grid), dim3(NTHREADS),
+    hipLaunchKernelGGL((batch_stage_kernel<D3, S2, R, VAH_BV_TX, VAH_BV_TY, B>), dim3(D3 ? L.grid3 : L.grid), dim3(NTHREADS),
                        0, L.stream, L.rec, D3 ? L.tiles3 : L.tiles2, L.counts);
 }
 

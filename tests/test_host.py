@@ -151,7 +151,7 @@ def test_batch_event_bytes_and_variants():
         assert V.batch_event_bytes(*dims) == expect(*dims), dims
     assert V.batch_event_bytes(0, 5, 5) == 0
     vs = V.batch_variants()
-    assert len(vs) >= 4 and vs[0] == (16, 15, 2, 8)
+    assert len(vs) >= 4 and vs[0] == (16, 15, 2, 3)
     for tx, ty, b, kz in vs:
         assert tx * ty <= 256 and (tx + 1) * ty <= 256 and tx * (ty + 1) <= 256
-        assert b in (1, 2, 3) and kz in (4, 8)
+        assert b in (1, 2, 3) and kz in (3, 6)

@@ -338,6 +338,47 @@ def serving_requests(V, n, seed):
     return reqs, base
 
 
+def bench_precision(V, args):
+    """Split precision against FP64, both on the GPU: the 81x81x1 optical
+    Glauber Au+Au event (b = 7 fm, conformal EOS, eta/s = 0.2, dtau = 0.01)
+    with seeded +-5% lognormal noise on e, 450 steps to tau = 5 fm/c.  Per-field
+    max and RMS relative errors (e; u^mu relative to u^tau; P_L; pi^munu
+    relative to e) and the drift of the total energy tau * sum T^tautau."""
+    nx = ny = 81
+    n = nx * ny
+    e = V.ic_optical_glauber(nx, ny, 1, 0.1, 0.1, V.glauber())
+    e = e * np.exp(0.05 * np.random.default_rng(2024).standard_normal(n))
+    steps = 450
+    runs = {}
+    for name, prec in (("split", V.FP32_SPLIT), ("fp64", V.FP64)):
+        g = V.Vah(nx, ny, 1, 0.1, 0.1, 0.1, 0.01, 0.2, 0.5, precision=prec)
+        g.set_primitives(e, np.ones(n), np.zeros(n), np.zeros(n), np.zeros(n), pl=e / 3)
+        g.synchronize()
+        a = time.perf_counter()
+        g.step(steps)
+        g.synchronize()
+        runs[name] = (g.state(), time.perf_counter() - a)
+        g.close()
+    a, b = runs["split"][0], runs["fp64"][0]
+    rel = {"e": np.abs(a["e"] - b["e"]) / b["e"]}
+    for d, k in enumerate(("u_tau", "u_x", "u_y", "u_eta")):
+        rel[k] = np.abs(a["u"][d] - b["u"][d]) / b["u"][0]
+    rel["P_L"] = np.abs(a["q"][4] - b["q"][4]) / b["q"][4]
+    rel["pi"] = np.max(np.abs(a["q"][5:] - b["q"][5:]), axis=0) / b["e"]
+    errs = {k: {"max": float(v.max()), "rms": float(np.sqrt(np.mean(v * v)))} for k, v in rel.items()}
+    drift = float(abs(np.sum(a["q"][0]) / np.sum(b["q"][0]) - 1.0))
+    worst = max(errs["e"]["max"], *(errs[k]["max"] for k in ("u_tau", "u_x", "u_y", "u_eta")))
+    print(json.dumps({
+        "metric": "max relative error of e and u^mu, split precision vs FP64", "value": worst, "unit": "relative",
+        "n_gpus": 1, "steps": steps, "warmup": 0, "ms_per_step": runs["split"][1] / steps * 1e3,
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32/f64", "data": "synthetic",
+        "config": {"workload": "2+1D optical Glauber Au+Au b=7 fm with seeded +-5% lognormal noise on e, 81x81x1, "
+                               "dx=0.1, tau0=0.5, dtau=0.01, 450 steps to tau=5 fm/c, conformal EOS, eta/s=0.2",
+                   "fp64_ms_per_step": runs["fp64"][1] / steps * 1e3},
+        "errors": errs, "total_energy_drift": drift,
+        "roofline": None, "cpu_baseline": None}))
+
+
 def bench_serving(V, args):
     serve = importlib.import_module("gpu-vah_amd.serve")
     tab = V.eos_lqcd_table(10000, 1e-8, 1e4)
@@ -436,9 +477,10 @@ def main():
                          "from step ~80 on (tau ~ 2.1 fm/c) and the event blows up near step 100, on the CPU "
                          "oracle as on the GPU")
     ap.add_argument("--warmup", type=int, default=5, help="untimed steps before the timed ones")
-    ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["batch", "serving"], default="mc3d",
+    ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["batch", "serving", "precision"], default="mc3d",
                     help="mc3d (default): one 201x201x101 MC-Glauber event; optical2d: 81x81x1 optical Glauber; "
-                         "batch: 256 boost-invariant events in one batch; serving: Poisson request stream")
+                         "batch: 256 boost-invariant events in one batch; serving: Poisson request stream; "
+                         "precision: split precision against FP64 on the noisy 81x81x1 event to tau = 5 fm/c")
     ap.add_argument("--events", type=int, default=256, help="batch / serving: number of events / requests")
     ap.add_argument("--autotune-steps", type=int, default=4, help="batch: steps per variant trial (0: no autotune)")
     ap.add_argument("--max-steps", type=int, default=1000,
@@ -469,6 +511,9 @@ def main():
         return
     if args.workload == "serving":
         bench_serving(V, args)
+        return
+    if args.workload == "precision":
+        bench_precision(V, args)
         return
     w = WORKLOADS[args.workload]
     e = initial_state(V, w)

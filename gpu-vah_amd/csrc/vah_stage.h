@@ -14,7 +14,8 @@
 // SHARED, each x or y face of the tile is solved once by one lane, its
 // 15-component flux parked in LDS and read by the two cells that share it;
 // the eta face between planes k and k+1 is solved by the cell's own lane and
-// carried in registers to the next plane.  Without SHARED every cell solves
+// carried to the next plane in an LDS slot only that lane touches (in
+// registers it would push the kernel into spilling).  Without SHARED every cell solves
 // both of its faces with its own Newton guess, the reference's work split.
 // blockIdx is remapped so that consecutive tiles land on the same XCD (the
 // hardware deals workgroups round-robin over the 8 XCDs), keeping tile halos

@@ -50,6 +50,17 @@ def test_3d_small(oracle_mod):
     assert_parity(a, b, tau, label="3D")
 
 
+@pytest.mark.parametrize("dims", [(17, 31, 2), (33, 16, 4), (16, 15, 7)], ids=["17x31x2", "33x16x4", "16x15x7"])
+def test_3d_ragged_tiles_and_eta_chunks(oracle_mod, dims):
+    """3+1D lattices whose sizes are not multiples of the 16 x 15 tile or of the
+    3-plane eta chunk (one chunk shorter than 3, a last chunk of 1 plane, a
+    single tile), tilted lumpy IC, 12 steps."""
+    nx, ny, nz = dims
+    e = gaussian_ic(nx, ny, nz, dz=0.4, sx=0.6, sy=0.9, tilt=0.4, seed=13)
+    a, b, tau = run_pair(oracle_mod, nx, ny, nz, 12, e, pl=e / 3, dz=0.4, dt=0.01)
+    assert_parity(a, b, tau, label="3D ragged")
+
+
 def test_flow_and_shear_initial_state(oracle_mod):
     """Initial flow and pi from an evolved oracle state, then both continue."""
     nx, ny, nz = 40, 36, 1

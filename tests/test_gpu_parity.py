@@ -222,3 +222,31 @@ def test_fp32_split_3d_against_fp64_oracle(oracle_mod):
         assert errs["e"][0] < 1e-4, label
         for k in ("ut", "ux", "uy", "un"):
             assert errs[k][0] < 1e-4, (label, k)
+
+
+@pytest.mark.parametrize("dims", [(17, 31, 2), (33, 16, 4)], ids=["17x31x2", "33x16x4"])
+def test_fp32_split_3d_ragged(oracle_mod, dims):
+    """The split-precision eta ring on lattices with a chunk shorter than 3
+    planes and a last chunk of one plane: e and u^mu within 1e-4 of the FP64
+    oracle after 20 steps."""
+    V = vah()
+    nx, ny, nz = dims
+    steps = 20
+    e = gaussian_ic(nx, ny, nz, dz=0.4, sx=0.6, sy=0.9, tilt=0.4, seed=17)
+    n = nx * ny * nz
+    o = oracle_mod.Oracle(nx, ny, nz, 0.1, 0.1, 0.4, 0.01, 0.2, 0.5)
+    o.set_primitives(e, *rest_frame(n), pl=e / 3)
+    o.step(steps)
+    b = o.state()
+    o.close()
+    g = V.Vah(nx, ny, nz, 0.1, 0.1, 0.4, 0.01, 0.2, 0.5, precision=V.FP32_SPLIT)
+    g.set_primitives(e, *rest_frame(n), pl=e / 3)
+    g.step(steps)
+    g.synchronize()
+    a = g.state()
+    g.close()
+    errs = precision_errors(a, b)
+    print(dims, errs)
+    assert errs["e"][0] < 1e-4
+    for k in ("ut", "ux", "uy", "un"):
+        assert errs[k][0] < 1e-4, k

@@ -68,6 +68,24 @@ __device__ __forceinline__ double rsqrt(double x) {
     return __builtin_fma(y, e, y);
 }
 
+// Square root of x >= 0: the device library's refinement of v_rsq_f64
+// (g = x r, h = r / 2, one Goldschmidt step, two residual corrections) without
+// its range scaling and special-operand fix-ups.  The scaling only acts below
+// 2^-767, so the result equals sqrt(x) for every x of at least that size; the
+// solver's arguments are 0, NaN or above 1e-40.  x = 0 gives 0, NaN gives NaN.
+__device__ __forceinline__ double sqrt_pos(double x) {
+    const double r = __builtin_amdgcn_rsq(x);
+    double g = x * r, h = 0.5 * r;
+    const double e = __builtin_fma(-h, g, 0.5);
+    g = __builtin_fma(g, e, g);
+    h = __builtin_fma(h, e, h);
+    double d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    return x == 0.0 ? 0.0 : g;
+}
+
 // ---------------------------------------------------------------------------
 // Equation of state
 // ---------------------------------------------------------------------------
@@ -163,9 +181,23 @@ __device__ __forceinline__ Powers powers_of(double a) {
     return w;
 }
 
+// A coefficient table in constant memory, addressed so that the compiler
+// reads it with scalar loads instead of folding its values into scalar moves
+// (two per coefficient and evaluation): the empty asm hides which table the
+// pointer holds.
+typedef __attribute__((address_space(4))) const double ConstTable;
+
+template <int N>
+__device__ __forceinline__ ConstTable *scalar_table(const double (&c)[N]) {
+    ConstTable *p = (ConstTable *)c;
+    __asm__("" : "+s"(p));
+    return p;
+}
+
 // sum c[i] * a^k[i], terms in the given order, constant term first
 template <int N>
-__device__ __forceinline__ double psum(const double (&c)[N], const int (&k)[N], const Powers &w) {
+__device__ __forceinline__ double psum(const double (&tab)[N], const int (&k)[N], const Powers &w) {
+    ConstTable *c = scalar_table(tab);
     double s = c[0];
 #pragma unroll
     for (int i = 1; i < N; ++i) s = s + c[i] * w.a[k[i]];
@@ -173,7 +205,8 @@ __device__ __forceinline__ double psum(const double (&c)[N], const int (&k)[N], 
 }
 
 template <int N>
-__device__ __forceinline__ double psum(const double (&c)[N], const Powers &w) {
+__device__ __forceinline__ double psum(const double (&tab)[N], const Powers &w) {
+    ConstTable *c = scalar_table(tab);
     double s = c[0];
 #pragma unroll
     for (int i = 1; i < N; ++i) s = s + c[i] * w.a[i];
@@ -184,44 +217,58 @@ struct Fits {
     double Rtilde, Rbar0, Rgamma, Rbar0P;
 };
 
+// Coefficients of the fits (numerator N, denominator D, constant term first).
+// They are read from constant memory, where each scalar load brings several
+// of them; as literals every one costs two scalar moves per evaluation.
+static __constant__ double kRbar0N[9] = {
+    0.015267955823446243, 7.725572805021035, 421.0063884634789, 3422.877939650926, -5785.670846299543,
+    -12261.66452089229, 31491.409484673808, -22737.05146992673, 5441.373392185447
+};
+static __constant__ double kRbar0D[9] = {
+    0.05470696094814806, 14.505878005231883, 522.6643024173569, 2731.7776413939037, -6161.1991042880445,
+    -3989.4375208972588, 15008.260526258282, -10243.036679405379, 2116.74060159494
+};
+static __constant__ double kRtildeN[8] = {
+    -6.674731906076046e-6, 0.004617789933500251, 0.7207562721999754, 9.097427250602184, -4.475814747302824,
+    -36.37501529319408, 46.868405146729316, -15.833867583743228
+};
+static __constant__ double kRtildeD[8] = {
+    0.06856675185266, 2.9181587012768597, 11.951184087839218, -29.708257843442173, -2.618233802059826,
+    34.646239784689065, -19.62596366454439, 2.374808442453899
+};
+static __constant__ double kRgammaN[8] = {
+    0.0001373796340585521, -0.6149634004722385, 3.1968875683514253, 246.35973783799196, -764.319750320186,
+    834.160165071214, -371.64955466673234, 52.87411921963021
+};
+static __constant__ double kRgammaD[8] = {
+    -1.673322188488071, 13.343782941396997, 561.7566534476223, -1790.2296622275915, 1896.4688704912812,
+    -658.7933063369629, -85.96181900698849, 65.09739194472589
+};
+static __constant__ double kRbar0PN[16] = {
+    2.473173363908116e-10, -4.899839370307281e-6, 155055.91462124084, -275435.45350226434, 350689.68825705117,
+    -299725.38986957155, 151477.08809203724, -33196.47417939176, -0.004301975027942015, -0.14858206981041563,
+    6.249255189587875, -92.79641927240235, 807.175057749925, -4760.015905266286, 20324.533122685436,
+    -64758.869552496515
+};
+static __constant__ double kRbar0PD[16] = {
+    0.00008222793468208523, 0.03411917870833943, 4.895969276094396e6, -8.84162305829353e6,
+    1.1445063656613324e7, -9.918442713390596e6, 5.065882388219598e6, -1.1181016364928822e6,
+    0.23871740573818725, -23.50912574236691, 417.4953123877312, -4234.215775452717, 29824.022790048104,
+    -157419.8447785501, 641300.6529027821, -2.0248032895288002e6
+};
+
 __device__ __forceinline__ double rbar0(const Powers &w) {
-    constexpr double n[9] = {0.015267955823446243, 7.725572805021035, 421.0063884634789, 3422.877939650926,
-                             -5785.670846299543, -12261.66452089229, 31491.409484673808, -22737.05146992673,
-                             5441.373392185447};
-    constexpr double d[9] = {0.05470696094814806, 14.505878005231883, 522.6643024173569, 2731.7776413939037,
-                             -6161.1991042880445, -3989.4375208972588, 15008.260526258282, -10243.036679405379,
-                             2116.74060159494};
-    return fdiv(psum(n, w), psum(d, w));
+    return fdiv(psum(kRbar0N, w), psum(kRbar0D, w));
 }
 
 __device__ __forceinline__ Fits fits(const Powers &w) {
-    constexpr double rt_n[8] = {-6.674731906076046e-6, 0.004617789933500251, 0.7207562721999754, 9.097427250602184,
-                                -4.475814747302824, -36.37501529319408, 46.868405146729316, -15.833867583743228};
-    constexpr double rt_d[8] = {0.06856675185266, 2.9181587012768597, 11.951184087839218, -29.708257843442173,
-                                -2.618233802059826, 34.646239784689065, -19.62596366454439, 2.374808442453899};
-    constexpr double rg_n[8] = {0.0001373796340585521, -0.6149634004722385, 3.1968875683514253, 246.35973783799196,
-                                -764.319750320186, 834.160165071214, -371.64955466673234, 52.87411921963021};
-    constexpr double rg_d[8] = {-1.673322188488071, 13.343782941396997, 561.7566534476223, -1790.2296622275915,
-                                1896.4688704912812, -658.7933063369629, -85.96181900698849, 65.09739194472589};
     // Rbar0P: the reference writes the terms in the order a^0, a^1, a^10..a^15, a^2..a^9
     constexpr int rp_k[16] = {0, 1, 10, 11, 12, 13, 14, 15, 2, 3, 4, 5, 6, 7, 8, 9};
-    constexpr double rp_n[16] = {2.473173363908116e-10, -4.899839370307281e-6, 155055.91462124084,
-                                 -275435.45350226434,   350689.68825705117,    -299725.38986957155,
-                                 151477.08809203724,    -33196.47417939176,    -0.004301975027942015,
-                                 -0.14858206981041563,  6.249255189587875,     -92.79641927240235,
-                                 807.175057749925,      -4760.015905266286,    20324.533122685436,
-                                 -64758.869552496515};
-    constexpr double rp_d[16] = {0.00008222793468208523, 0.03411917870833943, 4.895969276094396e6,
-                                 -8.84162305829353e6,    1.1445063656613324e7, -9.918442713390596e6,
-                                 5.065882388219598e6,    -1.1181016364928822e6, 0.23871740573818725,
-                                 -23.50912574236691,     417.4953123877312,    -4234.215775452717,
-                                 29824.022790048104,     -157419.8447785501,   641300.6529027821,
-                                 -2.0248032895288002e6};
     Fits f;
-    f.Rtilde = fdiv(psum(rt_n, w), psum(rt_d, w));
+    f.Rtilde = fdiv(psum(kRtildeN, w), psum(kRtildeD, w));
     f.Rbar0 = rbar0(w);
-    f.Rgamma = fdiv(psum(rg_n, w), psum(rg_d, w));
-    f.Rbar0P = fdiv(psum(rp_n, rp_k, w), psum(rp_d, rp_k, w));
+    f.Rgamma = fdiv(psum(kRgammaN, w), psum(kRgammaD, w));
+    f.Rbar0P = fdiv(psum(kRbar0PN, rp_k, w), psum(kRbar0PD, rp_k, w));
     return f;
 }
 
@@ -305,8 +352,8 @@ __device__ __forceinline__ Prim recover(const Eos &E, double t, const Q &q, doub
     if (M0PL == 0.0) M0PL = 1.e-16;
     double F = fdiv(M3, M0PL);
     double Ft = t * F;
-    double x = __builtin_sqrt(fabs(1.0 - Ft * Ft));
-    double MBT = __builtin_sqrt(M1 * M1 + M2 * M2);
+    double x = sqrt_pos(fabs(1.0 - Ft * Ft));
+    double MBT = sqrt_pos(M1 * M1 + M2 * M2);
     if (MBT == 0.0) MBT = 1.e-16;
     double uT = 0.0;
     Prim r;

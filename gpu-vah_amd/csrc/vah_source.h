@@ -285,11 +285,11 @@ __device__ __forceinline__ void regulate(double t, QA &q, double &e, double &p, 
     const double z0 = t * un * isF;
     const double z3 = ut / t * isF;
     const double t2 = t * t;
-    const double sTrsTrs = __builtin_sqrt(fabs(e * e + pl * pl + 2.0 * ptHat * ptHat));
+    const double sTrsTrs = sqrt_pos(fabs(e * e + pl * pl + 2.0 * ptHat * ptHat));
     const double pipi = pitt * pitt - 2.0 * pitx * pitx - 2.0 * pity * pity + pixx * pixx + 2.0 * pixy * pixy +
                         piyy * piyy - 2.0 * pitn * pitn * t2 + 2.0 * pixn * pixn * t2 + 2.0 * piyn * piyn * t2 +
                         pinn * pinn * t2 * t2;
-    const double spipi = __builtin_sqrt(fabs(pipi));
+    const double spipi = sqrt_pos(fabs(pipi));
     const double ar[9] = {pitt - pixx - piyy - pinn * t2,
                           -pitn * t2 * un + pitt * ut - pitx * ux - pity * uy,
                           -pixn * t2 * un + pitx * ut - pixx * ux - pixy * uy,

@@ -98,8 +98,8 @@ __device__ __forceinline__ FaceState face_state(double t, double ttt, double ttx
     if (M0PL == 0.0) M0PL = 1.e-16;
     f.F = fdiv(f.M3, M0PL);
     f.Ft = t * f.F;
-    f.x = __builtin_sqrt(fabs(1.0 - f.Ft * f.Ft));
-    f.MBT = __builtin_sqrt(f.M1 * f.M1 + f.M2 * f.M2);
+    f.x = sqrt_pos(fabs(1.0 - f.Ft * f.Ft));
+    f.MBT = sqrt_pos(f.M1 * f.M1 + f.M2 * f.M2);
     if (f.MBT == 0.0) f.MBT = 1.e-16;
     return f;
 }
@@ -276,7 +276,7 @@ __device__ __forceinline__ void face_flux(double t, Guess guess, const Buf &q, u
 
 __device__ __forceinline__ double uT_of(const Buf &u, uint32_t f8, uint32_t c) {
     double ux = ld(u, c, f8), uy = ld(u, c, 2 * f8);
-    return __builtin_sqrt(fabs(ux * ux + uy * uy));
+    return sqrt_pos(fabs(ux * ux + uy * uy));
 }
 
 // d_i of the pi components entering the axis source of direction DIR
@@ -496,8 +496,8 @@ __device__ __forceinline__ void stage_tile(const StageParams &P, int tile, R (*H
             // that seeds it, u_cur in stage 1 and u_prev in stage 2
             // (FullyDiscreteKurganovTadmorScheme.cpp:761-780)
             rec.e_prev = in.e;
-            rec.uT_guess = STAGE2 ? __builtin_sqrt(fabs(in.uxp * in.uxp + in.uyp * in.uyp))
-                                  : __builtin_sqrt(fabs(in.ux * in.ux + in.uy * in.uy));
+            rec.uT_guess = STAGE2 ? sqrt_pos(fabs(in.uxp * in.uxp + in.uyp * in.uyp))
+                                  : sqrt_pos(fabs(in.ux * in.ux + in.uy * in.uy));
             iut = rcp(in.ut);
         }
 

@@ -285,21 +285,36 @@ __device__ __forceinline__ double pt_hat(double e, double p, double pl) {
 // a = y - x and b = z - y are nonzero with one sign; z - x then has that sign
 // too, and the result is the smallest of |l|, |c|, |r| with that sign.  In
 // every other case -- opposite signs, or a zero, whose sign the reference
-// counts as +1 -- one of the minima is 0.  Here min(|l|, |r|) is formed as
-// theta * min(|a|, |b|), equal because rounding is monotonic, and "nonzero
-// with one sign" as a * b > 0.  That test differs only when a * b underflows
-// to 0; the slope is then 0 instead of a value of at most
-// 1.1 * sqrt(5e-324) = 2.4e-162 in magnitude (2.6e-23 in float).
+// counts as +1 -- one of the minima is 0.
 // half_slope returns half of it, the step from the cell to its face: halving
-// commutes with rounding, so (theta / 2) * min(|a|, |b|) and |z - x| / 4 give
-// the halves of the two candidates exactly (short of subnormal results).
+// commutes with rounding, so (theta / 2) a, (theta / 2) b and (z - x) / 4 are
+// the halves of the candidates exactly (short of subnormal results), and
+// min(|l|, |r|) / 2 is (theta / 2) min(|a|, |b|) because rounding is monotonic.
 // ---------------------------------------------------------------------------
-template <typename R>
-__device__ __forceinline__ R half_slope(R x, R y, R z) {
-    constexpr R kHalfTheta = R(1.1) * R(0.5);
-    const R a = y - x, b = z - y;
-    const R m = fmin(kHalfTheta * fmin(fabs(a), fabs(b)), R(0.25) * fabs(z - x));
-    return a * b > R(0) ? __builtin_copysign(m, a) : R(0);
+
+// Double: "nonzero with one sign" is tested as a * b > 0.  That test differs
+// from the reference's only when a * b underflows to 0; the slope is then 0
+// instead of a value of at most 1.1 * sqrt(5e-324) = 2.4e-162 in magnitude.
+// (Carrying the sign through min / max instead costs as many instructions:
+// the compiler canonicalises a copysign result before v_min_f64.)
+__device__ __forceinline__ double half_slope(double x, double y, double z) {
+    constexpr double kHalfTheta = 1.1 * 0.5;
+    const double a = y - x, b = z - y;
+    const double m = fmin(kHalfTheta * fmin(fabs(a), fabs(b)), 0.25 * fabs(z - x));
+    return a * b > 0.0 ? __builtin_copysign(m, a) : 0.0;
+}
+
+// Float, on the hardware median: the median of u, v and 0 is minmod(u, v)
+// (0 when the signs differ, else the one nearer 0), so half the slope is
+// med3(l, med3(c, r, 0), 0) with the halved candidates l = (theta / 2) a,
+// c = (z - x) / 4, r = (theta / 2) b -- two v_med3_f32, and l of one face
+// side is r of the other.  Equal to the double form's arithmetic done in float
+// wherever a * b does not underflow (|a|, |b| below ~4e-23), where this one
+// keeps the reference's tiny slope.
+__device__ __forceinline__ float half_slope(float x, float y, float z) {
+    constexpr float kHalfTheta = 1.1f * 0.5f;
+    const float l = kHalfTheta * (y - x), r = kHalfTheta * (z - y), c = 0.25f * (z - x);
+    return __builtin_amdgcn_fmed3f(l, __builtin_amdgcn_fmed3f(c, r, 0.0f), 0.0f);
 }
 
 // ---------------------------------------------------------------------------

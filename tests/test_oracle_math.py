@@ -107,30 +107,46 @@ def test_slope_closed_form(oracle_mod):
 
 
 def _device_half_slope(x, y, z):
-    """The stage kernels' form of half the minmod slope (half_slope in
+    """The stage kernels' double form of half the minmod slope (half_slope in
     gpu-vah_amd/csrc/vah_device.h), in numpy double arithmetic."""
     a, b = y - x, z - y
     m = np.minimum((1.1 * 0.5) * np.minimum(np.abs(a), np.abs(b)), 0.25 * np.abs(z - x))
     return np.where(a * b > 0, np.copysign(m, a), 0.0)
 
 
-def test_device_half_slope_is_half_the_reference_slope(oracle_mod):
-    """The kernels reconstruct faces with theta/2 * min(|a|, |b|), |z - x| / 4
-    and a sign test a * b > 0 in place of the reference's sign()-based minmod
-    chain (FluxLimiter.cpp:13-33, which the oracle reproduces bit for bit):
-    equal to half the reference slope bit for bit over twelve decades of
-    scale, every sign pattern, zeros and ties; where a * b underflows the
-    kernels' slope is 0 and the reference's below 2.4e-162."""
-    L = oracle_mod.lib()
-    rng = np.random.default_rng(12)
-    n = 20000
-    scale = 10.0 ** rng.uniform(-150, 150, n)
-    x, y, z = (rng.normal(0, 1, (3, n)) * scale)
+def _med3(u, v, w):
+    return np.maximum(np.minimum(u, v), np.minimum(np.maximum(u, v), w))
+
+
+def _device_half_slope_f32(x, y, z):
+    """The float form: med3(l, med3(c, r, 0), 0) of the halved candidates."""
+    h = np.float32(1.1) * np.float32(0.5)
+    zero = np.float32(0.0)
+    l, r, c = h * (y - x), h * (z - y), np.float32(0.25) * (z - x)
+    return _med3(l, _med3(c, r, zero), zero)
+
+
+def _scaled_triples(rng, n, lo, hi, dtype):
+    scale = 10.0 ** rng.uniform(lo, hi, n)
+    x, y, z = (rng.normal(0, 1, (3, n)) * scale).astype(dtype)
     # ties and zero differences
     y[:500] = x[:500]
     z[500:1000] = y[500:1000]
     z[1000:1500] = x[1000:1500]
     x[1500:2000] = 0.0
+    return x, y, z
+
+
+def test_device_half_slope_is_half_the_reference_slope(oracle_mod):
+    """The kernels reconstruct faces with theta/2 * min(|a|, |b|), |z - x| / 4
+    and a sign test a * b > 0 in place of the reference's sign()-based minmod
+    chain (FluxLimiter.cpp:13-33, which the oracle reproduces bit for bit):
+    equal to half the reference slope bit for bit over three hundred decades
+    of scale, every sign pattern, zeros and ties; where a * b underflows the
+    kernels' slope is 0 and the reference's below 2.4e-162."""
+    L = oracle_mod.lib()
+    rng = np.random.default_rng(12)
+    x, y, z = _scaled_triples(rng, 20000, -150, 150, np.float64)
     got = _device_half_slope(x, y, z)
     want = np.array([0.5 * L.orc_slope(*v) for v in zip(x, y, z)])
     assert np.array_equal(got, want)
@@ -138,7 +154,26 @@ def test_device_half_slope_is_half_the_reference_slope(oracle_mod):
     x, y, z = np.array([0.0, 0.0, 1e-170]), np.array([1e-170, -1e-170, 3e-170]), np.array([3e-170, -5e-170, 4e-170])
     want = np.array([0.5 * L.orc_slope(*v) for v in zip(x, y, z)])
     got = _device_half_slope(x, y, z)
-    assert np.all(got == 0.0) and np.all(np.abs(want) < 2.4e-162) and np.any(want != 0.0)
+    assert np.all(got == 0.0) and np.all(np.abs(want) < 2.4e-162) and np.all(want != 0.0)
+
+
+def test_float_half_slope_median_form():
+    """The split-precision kernels' float slope, med3(l, med3(c, r, 0), 0),
+    against the sign-test form min(theta/2 min(|a|, |b|), |z - x| / 4) with the
+    sign of a when a * b > 0, in float32: equal wherever a * b does not
+    underflow, and the median form is the minmod of its three candidates."""
+    rng = np.random.default_rng(31)
+    x, y, z = _scaled_triples(rng, 20000, -15, 15, np.float32)
+    got = _device_half_slope_f32(x, y, z)
+    a, b = y - x, z - y
+    m = np.minimum(np.float32(0.55) * np.minimum(np.abs(a), np.abs(b)), np.float32(0.25) * np.abs(z - x))
+    want = np.where(a * b > 0, np.copysign(m, a), np.float32(0.0))
+    assert got.dtype == np.float32 and np.array_equal(got, want)
+    # minmod of the three halved candidates, straight from its definition
+    cand = np.stack([np.float32(0.55) * a, np.float32(0.25) * (z - x), np.float32(0.55) * b])
+    same = np.all(cand > 0, axis=0) | np.all(cand < 0, axis=0)
+    pick = np.take_along_axis(cand, np.argmin(np.abs(cand), axis=0)[None], axis=0)[0]
+    assert np.array_equal(got, np.where(same, pick, np.float32(0.0)))
 
 
 def test_conformal_temperature_uses_single_precision_powf(oracle_mod):

@@ -12,7 +12,7 @@ default 5 warmup + 50 timed steps stay in the part of the evolution where
 every cell's Newton recovery succeeds; longer runs restart the event every 60
 steps, outside the timed region (see the --steps help).
 
-Prints one JSON line.  --gpus above 1 is refused: this build runs on one GPU.
+Prints one JSON line.  --gpus above 1 is refused: the benchmark runs on one GPU.
 """
 import argparse
 import importlib
@@ -500,8 +500,9 @@ def main():
                     help="only run a plain copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(a multi-GPU run would shard events or eta slabs, see the design notes)", file=sys.stderr)
+        print(f"bench.py: the step runs on one GPU; --gpus {args.gpus} is not supported "
+              "(several GPUs would shard by event, or a single event by eta slabs with a halo exchange per stage)",
+              file=sys.stderr)
         sys.exit(2)
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)

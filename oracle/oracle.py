@@ -1,7 +1,7 @@
 """ctypes front end of the CPU oracle (oracle/vah_oracle.c).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, by ``__graft_entry__.smoke()``
-as the checker and by ``bench.py``'s CPU-baseline leg.  The product path never
+for the expected answer and by ``bench.py``'s CPU-baseline leg.  The product path never
 imports it.
 
 The oracle restates the reference's VAH Heun-RK2 step

@@ -2,7 +2,7 @@
  * vah_oracle.c -- CPU restatement of the reference's anisotropic viscous
  * hydrodynamics (VAH) Kurganov-Tadmor Heun-RK2 time step.
  *
- * TEST INFRASTRUCTURE ONLY.  This file is the checker: the parity tests, the
+ * TEST INFRASTRUCTURE ONLY.  This file is the oracle: the parity tests, the
  * smoke check and the CPU-baseline leg of the benchmark load it to produce the
  * expected answer or a CPU timing.  The product (gpu-vah_amd/) never links,
  * loads or calls it.

@@ -106,6 +106,9 @@ typedef struct vah_profile {
 
 typedef struct vah_ctx vah_ctx;
 
+/* Default options: conformal EOS, FP64, shared faces, library default eta
+ * chunk, the context's own stream.  No counterpart: the reference fixes these
+ * at compile time (CONFORMAL_EOS, PRECISION, CudaConfiguration.cu). */
 void vah_default_options(vah_options *opt);
 
 /* Allocates the device state for one lattice: 3 x 15 conserved fields, e and
@@ -123,7 +126,9 @@ int vah_set_primitives(vah_ctx *ctx, const double *e, const double *pl, const do
                        const double *uy, const double *un, const double *pi);
 
 /* Exact state load: q (15 field-major), e, p, u and up (4 field-major each:
- * u^tau, u^x, u^y, u^eta); tau = t0 + nstep*dt.  Ghosts are filled on device. */
+ * u^tau, u^x, u^y, u^eta); tau = t0 + nstep*dt.  Ghosts are filled on device.
+ * No counterpart (the reference only starts from primitives): restarts from a
+ * saved state and the parity tests' oracle states. */
 int vah_set_state(vah_ctx *ctx, long long nstep, const double *q, const double *e, const double *p, const double *u,
                   const double *up);
 
@@ -132,19 +137,25 @@ int vah_set_state(vah_ctx *ctx, long long nstep, const double *q, const double *
  * Asynchronous on the context's stream. */
 int vah_step(vah_ctx *ctx, int nsteps);
 
-/* Blocks until the context's stream is idle; reports NaN cells as VAH_ERR_NAN. */
+/* Blocks until the context's stream is idle; reports NaN cells as VAH_ERR_NAN
+ * (the cudaDeviceSynchronize that ends each reference step,
+ * rhic/src/FullyDiscreteKurganovTadmorScheme.cu:156; the CPU path exits on NaN
+ * instead, old_cpu-vah_code/src/EnergyMomentumTensor.cpp:193). */
 int vah_synchronize(vah_ctx *ctx);
 
 /* Copies the current state to the host (copyDeviceToHostMemory).  Any pointer
  * may be NULL.  Same layouts as vah_set_state. */
 int vah_get_state(vah_ctx *ctx, double *q, double *e, double *p, double *u, double *up);
 
+/* tau = t0 + nstep * dt and nstep, the driver's clock (rhic/src/HydroPlugin.cpp:232). */
 int vah_get_time(const vah_ctx *ctx, double *tau, long long *nstep);
 
-/* Device-side reductions over interior cells. */
+/* Device-side reductions over interior cells (max e, max T, tau sum T^tautau,
+ * max |v_T|, counters).  Replaces what run() reads from a full host copy of
+ * the state: the centre-cell printout and the freeze-out test
+ * (rhic/src/HydroPlugin.cpp:156-158,198-219). */
 int vah_reduce(vah_ctx *ctx, vah_stats *out);
 
-/* Per-launch HIP-event timing of the two stage kernels (off by default). */
 /* First-order validity diagnostics after a step, per interior cell: the
  * Knudsen numbers 5 (eta/s) |theta_T| / T, 5 (eta/s) |zDzu| / T,
  * 5 (eta/s) |theta| / T and the inverse Reynolds number sqrt|pi:pi| / |P_T hat|
@@ -155,15 +166,20 @@ int vah_reduce(vah_ctx *ctx, vah_stats *out);
  * may be NULL; valid until the next vah_step. */
 int vah_validity(vah_ctx *ctx, double *kn_T, double *kn_L, double *kn, double *r_pi);
 
+/* Per-launch HIP-event timing of the two stage kernels (off by default); the
+ * reference times each step with a host stopwatch (rhic/src/HydroPlugin.cpp:222-227,234). */
 int vah_profile_enable(vah_ctx *ctx, int on);
 int vah_profile_read(vah_ctx *ctx, vah_profile *out);
 
+/* Frees the context: freeHostMemory + freeDeviceMemory (rhic/src/HydroPlugin.cpp:239-240,
+ * rhic/include/DynamicalVariables.cuh:145-181). */
 void vah_destroy(vah_ctx *ctx);
 
-/* Text of the last error on this thread. */
+/* Text of the last error on this thread.  No counterpart: the reference prints
+ * and exits where this library returns a status. */
 const char *vah_last_error(void);
 
-/* Build identification: "gfx950 <precision modes> <compiler>". */
+/* Build identification: "gfx950 <precision modes> <compiler>".  No counterpart. */
 const char *vah_build_info(void);
 
 #ifdef __cplusplus

@@ -25,7 +25,8 @@ extern "C" {
  * [e_min, e_max] (fm^-4): p[i], T[i] at ln e_i = *lne0 + i * *h. */
 int vah_eos_lqcd_table(int n, double e_min, double e_max, double *p, double *T, double *lne0, double *h);
 
-/* The crossover EOS evaluated directly at temperature T (fm^-1): e and p. */
+/* The crossover EOS evaluated directly at temperature T (fm^-1): e and p.
+ * Not in the reference (its table generator's closed form). */
 void vah_eos_lqcd_eval(double T, double *e, double *p);
 
 /* ---- Initial conditions ------------------------------------------------ */
@@ -48,10 +49,14 @@ typedef struct vah_glauber {
     double e_floor;           /* added to every cell [fm^-4] (the reference adds 1e-3) */
 } vah_glauber;
 
-/* Optical (continuous) Glauber: interior e array (x fastest). */
+/* Optical (continuous) Glauber: interior e array (x fastest).  Replaces
+ * rhic/src/GlauberModel.cpp:19-108, with an own quadrature of the thickness
+ * integral in place of GSL's QAGS (GlauberModel.cpp:36-50). */
 int vah_ic_optical_glauber(const vah_lattice *lat, const vah_glauber *g, double *e);
 
-/* Monte-Carlo Glauber with this library's seeded generator (xoshiro256**):
+/* Monte-Carlo Glauber (rhic/src/MonteCarloGlauberModel.cpp:17-108, which
+ * draws from the C library's rand()) with this library's seeded generator
+ * (xoshiro256**):
  * nucleons sampled from Woods-Saxon, wounded by the black-disk criterion
  * d < sqrt(sigma/pi), each depositing a normalised-to-peak Gaussian.
  * n_wounded may be NULL. */
@@ -69,11 +74,11 @@ typedef struct vah_config {
     /* lattice.properties (rhic/src/LatticeParameters.cpp:30-39) */
     int numLatticePointsX, numLatticePointsY, numLatticePointsRapidity, numProperTimePoints;
     double latticeSpacingX, latticeSpacingY, latticeSpacingRapidity, latticeSpacingProperTime;
-    /* ic.properties (rhic/src/InitialConditionParameters.cpp:86-94) */
+    /* ic.properties (rhic/src/InitialConditionParameters.cpp:33-41) */
     int initialConditionType, numberOfNucleonsPerNuclei;
     double initialEnergyDensity, scatteringCrossSectionNN, impactParameter, fractionOfBinaryCollisions;
     double rapidityVariance, rapidityMean;
-    /* hydro.properties (rhic/src/HydroParameters.cpp:130-134) */
+    /* hydro.properties (rhic/src/HydroParameters.cpp:25-29) */
     double initialProperTimePoint, shearViscosityToEntropyDensity, freezeoutTemperatureGeV;
     int initializePimunuNavierStokes;
 } vah_config;

@@ -92,7 +92,7 @@ def test_eta_profile():
 def test_config_defaults_and_reference_files(tmp_path):
     V = vah()
     c = V.config_load(str(tmp_path / "missing"))
-    # LatticeParameters.cpp:30-39, InitialConditionParameters.cpp:86-94, HydroParameters.cpp:130-134
+    # LatticeParameters.cpp:30-39, InitialConditionParameters.cpp:33-41, HydroParameters.cpp:25-29
     assert (c.numLatticePointsX, c.numLatticePointsY, c.numLatticePointsRapidity, c.numProperTimePoints) == \
         (128, 128, 64, 10)
     assert (c.latticeSpacingX, c.latticeSpacingRapidity, c.latticeSpacingProperTime) == (0.08, 0.3, 0.01)

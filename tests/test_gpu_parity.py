@@ -43,10 +43,12 @@ def test_reference_pl_init_and_lumps(oracle_mod):
     assert_parity(a, b, tau, label="lumpy")
 
 
-def test_3d_small(oracle_mod):
-    """3+1D 24x20x14 with an eta plateau, 25 steps."""
+@pytest.mark.parametrize("flux_mode", [0, 1], ids=["shared", "per_cell"])
+def test_3d_small(oracle_mod, flux_mode):
+    """3+1D 24x20x14 with an eta plateau, 25 steps, in both flux modes (faces
+    shared between cells, and the reference's per-cell work split)."""
     e = gaussian_ic(24, 20, 14, dz=0.3, seed=9)
-    a, b, tau = run_pair(oracle_mod, 24, 20, 14, 25, e, pl=e / 3, dz=0.3, dt=0.01)
+    a, b, tau = run_pair(oracle_mod, 24, 20, 14, 25, e, pl=e / 3, dz=0.3, dt=0.01, flux_mode=flux_mode)
     assert_parity(a, b, tau, label="3D")
 
 

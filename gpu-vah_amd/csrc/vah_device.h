@@ -71,8 +71,10 @@ __device__ __forceinline__ double rsqrt(double x) {
 // Square root of x >= 0: the device library's refinement of v_rsq_f64
 // (g = x r, h = r / 2, one Goldschmidt step, two residual corrections) without
 // its range scaling and special-operand fix-ups.  The scaling only acts below
-// 2^-767, so the result equals sqrt(x) for every x of at least that size; the
-// solver's arguments are 0, NaN or above 1e-40.  x = 0 gives 0, NaN gives NaN.
+// 2^-767, so the result equals sqrt(x) for every x of at least that size (the
+// solver's arguments are squares and sums of squares of physical quantities);
+// below it, down to the smallest normal x, the refinement still converges
+// (g^2 stays normal).  x = 0 gives 0, NaN gives NaN.
 __device__ __forceinline__ double sqrt_pos(double x) {
     const double r = __builtin_amdgcn_rsq(x);
     double g = x * r, h = 0.5 * r;

@@ -20,6 +20,7 @@
 
 #include "vah_eos_wb.h"
 #include "vah_log.h"
+#include "vah_sqrt.h"
 #include "vah_tanh.h"
 
 namespace vah {
@@ -68,25 +69,12 @@ __device__ __forceinline__ double rsqrt(double x) {
     return __builtin_fma(y, e, y);
 }
 
-// Square root of x >= 0: the device library's refinement of v_rsq_f64
-// (g = x r, h = r / 2, one Goldschmidt step, two residual corrections) without
-// its range scaling and special-operand fix-ups.  The scaling only acts below
-// 2^-767, so the result equals sqrt(x) for every x of at least that size (the
-// solver's arguments are squares and sums of squares of physical quantities);
-// below it, down to the smallest normal x, the refinement still converges
-// (g^2 stays normal).  x = 0 gives 0, NaN gives NaN.
-__device__ __forceinline__ double sqrt_pos(double x) {
-    const double r = __builtin_amdgcn_rsq(x);
-    double g = x * r, h = 0.5 * r;
-    const double e = __builtin_fma(-h, g, 0.5);
-    g = __builtin_fma(g, e, g);
-    h = __builtin_fma(h, e, h);
-    double d = __builtin_fma(-g, g, x);
-    g = __builtin_fma(d, h, g);
-    d = __builtin_fma(-g, g, x);
-    g = __builtin_fma(d, h, g);
-    return x == 0.0 ? 0.0 : g;
-}
+// Square root of x >= 0 from the hardware reciprocal square root and the
+// refinement of vah_sqrt.h: the device library's sequence without its range
+// scaling, which only acts below 2^-767, so the result equals sqrt(x) for every
+// x of at least that size (the solver's arguments are squares and sums of
+// squares of physical quantities).  x = 0 gives 0, NaN gives NaN.
+__device__ __forceinline__ double sqrt_pos(double x) { return sqrt_refine(x, __builtin_amdgcn_rsq(x)); }
 
 // ---------------------------------------------------------------------------
 // Equation of state

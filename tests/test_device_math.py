@@ -72,3 +72,49 @@ def test_eos_log_and_tanh_ratio_against_libm(tmp_path):
     worst_log, special_log, worst_tanh, special_tanh = map(int, out)
     assert special_log == 1 and worst_log <= 2
     assert special_tanh == 1 and worst_tanh <= 6
+
+
+SQRT_SRC = r'''
+#define __host__
+#define __device__
+#include SQRT_H
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <random>
+static int64_t bits(double x) { int64_t i; std::memcpy(&i, &x, 8); return i; }
+int main() {
+    std::mt19937_64 g(17);
+    std::uniform_real_distribution<double> ex(-760.0, 1000.0), m(1.0, 2.0), d(-1.0, 1.0);
+    long long worst = 0, exact = 0;
+    const int n = 2000000;
+    for (int k = 0; k < n; ++k) {
+        const double x = std::ldexp(m(g), (int)std::floor(ex(g)));
+        const double seed = (1.0 / std::sqrt(x)) * (1.0 + std::ldexp(d(g), -20));  // a seed off by < 2^-20
+        const long long e = std::llabs(bits(vah::sqrt_refine(x, seed)) - bits(std::sqrt(x)));
+        if (e > worst) worst = e;
+        if (e == 0) ++exact;
+    }
+    const bool special = vah::sqrt_refine(0.0, INFINITY) == 0.0 && std::isnan(vah::sqrt_refine(NAN, NAN)) &&
+                         vah::sqrt_refine(1.0, 1.0) == 1.0 && vah::sqrt_refine(4.0, 0.5) == 2.0;
+    std::printf("%lld %lld %d %d\n", worst, exact, n, special ? 1 : 0);
+    return 0;
+}
+'''
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++")
+def test_sqrt_refinement_is_correctly_rounded(tmp_path):
+    """The refinement behind the kernels' square roots (sqrt_refine in
+    gpu-vah_amd/csrc/vah_sqrt.h, seeded on the device by v_rsq_f64): from seeds
+    off by up to 2^-20 it returns the C library's correctly rounded sqrt bit for
+    bit over 2^-760 .. 2^1000, and 0 at 0, NaN at NaN."""
+    src = tmp_path / "s.cpp"
+    src.write_text(SQRT_SRC.replace("SQRT_H", '"' + os.path.join(CSRC, "vah_sqrt.h") + '"'))
+    exe = tmp_path / "s"
+    subprocess.run(["g++", "-O2", "-ffp-contract=off", "-o", str(exe), str(src)], check=True)
+    worst, exact, n, special = map(int, subprocess.run([str(exe)], check=True, capture_output=True,
+                                                      text=True).stdout.split())
+    assert special == 1
+    assert worst == 0 and exact == n

@@ -22,7 +22,7 @@ FIELDS = ("ttt", "ttx", "tty", "ttn", "pl", "pitt", "pitx", "pity", "pitn",
           "pixx", "pixy", "pixn", "piyy", "piyn", "pinn")
 
 VAH_OK = 0
-VAH_ERR_NAN = -4
+VAH_ERR_ARG, VAH_ERR_HIP, VAH_ERR_NOMEM, VAH_ERR_NAN, VAH_ERR_STATE, VAH_ERR_DEVICE = -1, -2, -3, -4, -5, -6
 VAH_ERR_BUDGET = -7
 VAH_ERR_FULL = -8
 EVENT_LIVE, EVENT_FROZEN, EVENT_STEP_LIMIT, EVENT_NAN = 0, 1, 2, 3

@@ -155,3 +155,26 @@ def test_batch_event_bytes_and_variants():
     for tx, ty, b, kz in vs:
         assert tx * ty <= 256 and (tx + 1) * ty <= 256 and tx * (ty + 1) <= 256
         assert b in (1, 2, 3) and kz in (3, 6)
+
+
+@pytest.mark.parametrize("kw, msg", [
+    (dict(nx=0), "lattice sizes"),
+    (dict(dx=-0.1), "lattice sizes"),
+    (dict(t0=0.0), "t0 > 0"),
+    (dict(etabar=0.0), "eta/s > 0"),
+    (dict(precision=7), "unknown precision"),
+    (dict(precision=1, flux_mode=1), "shared faces only"),
+    (dict(eta_chunk=300), "eta_chunk"),
+    (dict(flux_mode=5), "unknown flux mode"),
+    (dict(eos_kind=9), "unknown EOS kind"),
+    (dict(eos_table=(0.0, 0.0, [1.0, 2.0], [1.0, 2.0])), "EOS table"),
+], ids=["nx", "dx", "t0", "etabar", "precision", "split-per-cell", "eta_chunk", "flux_mode", "eos_kind", "table_h"])
+def test_vah_create_rejects_bad_arguments_before_touching_the_device(kw, msg):
+    """vah_create validates its arguments before any HIP call: each bad value
+    comes back as VAH_ERR_ARG with its message, here without a GPU."""
+    V = vah()
+    args = dict(nx=10, ny=10, nz=1, dx=0.1, dy=0.1, dz=0.1, dt=0.01, etabar=0.2, t0=0.5)
+    args.update(kw)
+    with pytest.raises(V.VahError) as exc:
+        V.Vah(**args)
+    assert exc.value.code == V.VAH_ERR_ARG and msg in str(exc.value)

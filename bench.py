@@ -34,10 +34,11 @@ BYTES_PER_CELL_STAGE = (368, 488)  # algorithmic bytes per cell of the stage-1 /
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "stage_kernel_hbm_traffic_201x201x101_fp64.md")
 
 WORKLOADS = {
-    "mc3d": dict(nx=201, ny=201, nz=101, dx=0.1, dz=0.1, dt=0.02, t0=0.5, etabar=0.16, eos="table", ic="mc",
+    "mc3d": dict(nx=201, ny=201, nz=101, dx=0.1, dz=0.1, dt=0.02, t0=0.5, etabar=0.16, eos="table", ic="mc", restart=60,
                name="3+1D MC-Glauber Pb+Pb b=6 fm, 201x201x101, dx=deta=0.1, tau0=0.5, dtau=0.02, "
                     "LQCD-like EOS table (1e4 nodes), eta/s=0.16"),
     "optical2d": dict(nx=81, ny=81, nz=1, dx=0.1, dz=0.1, dt=0.01, t0=0.5, etabar=0.2, eos="conformal", ic="optical",
+                      restart=None,
                name="2+1D optical Glauber Au+Au b=7 fm, 81x81x1, dx=0.1, tau0=0.5, dtau=0.01, conformal EOS, "
                     "eta/s=0.2"),
 }
@@ -527,10 +528,11 @@ def main():
               eos_table=eos_table, flux_mode=mode,
               precision=V.FP32_SPLIT if args.precision == "fp32_split" else V.FP64, eta_chunk=args.eta_chunk)
     # The 201x201x101 event's 0.5 fm hot spots break the scheme down at the
-    # dilute edge from step ~80 on (the CPU oracle does the same), so the event
+    # dilute edge from step ~80 on (the CPU oracle does the same), so that event
     # is restarted from its initial state every reset_steps steps, outside the
     # timed region: runs of any length time steps of the healthy evolution.
-    reset_steps = 60
+    # Other workloads run as one evolution.
+    reset_steps = w["restart"] or (args.warmup + args.steps + 1)
     age = [reset_steps]
 
     def run(k):
@@ -582,7 +584,7 @@ def main():
         "dtype": "f64" if args.precision == "fp64" else "f32/f64",
         "data": "synthetic",
         "config": {"workload": w["name"], "cells": n, "flux_mode": args.flux_mode,
-                   "precision": args.precision, "restart_every_steps": reset_steps,
+                   "precision": args.precision, "restart_every_steps": w["restart"],
                    "stage_kernel_ms": prof["avg_ms"], "recovery_fallbacks": st.recovery_fallbacks,
                    "nan_cells": st.nan_cells},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

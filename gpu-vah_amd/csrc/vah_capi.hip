@@ -161,6 +161,7 @@ struct vah_ctx {
     vah_host::Layout lay;
     long long nstep;
     bool have_state;
+    bool stepped;          // a step has completed since the last vah_set_state
     // launch geometry
     int tiles_x, tiles_y, zchunks, kz, nblocks, grid;
     // profiling
@@ -323,7 +324,8 @@ __global__ void reduce_kernel(const double *q, const double *e, const double *u,
 }
 
 // First-order validity diagnostics (HydrodynamicValidity.cpp:47-247) with
-// the reference's arguments after a step: t_n, the conserved variables q^n of
+// the reference's arguments after a step: t_{n+1} = t_n + dt (rungeKutta2
+// advances t before its checkValidity call), the conserved variables q^n of
 // the step's start, the new e, p, u and u^n as the previous velocity.
 // out[f*n + m] for f = kn_T, kn_L, kn, r_pi (m: interior cell, x fastest).
 __global__ void validity_kernel(const double *qn, const double *e, const double *p, const double *u,
@@ -549,6 +551,7 @@ int vah_set_state(vah_ctx *c, long long nstep, const double *q, const double *e,
     HIP_TRY(hipStreamSynchronize(c->stream));
     c->nstep = nstep;
     c->have_state = true;
+    c->stepped = false;
     return VAH_OK;
 }
 
@@ -583,6 +586,7 @@ int vah_step(vah_ctx *c, int nsteps) {
         c->uc = c->up;
         c->up = old_uc;
         c->nstep++;
+        c->stepped = true;
     }
     return VAH_OK;
 }
@@ -653,12 +657,15 @@ int vah_reduce(vah_ctx *c, vah_stats *out) {
 
 int vah_validity(vah_ctx *c, double *kn_T, double *kn_L, double *kn, double *r_pi) {
     if (!c) return fail(VAH_ERR_ARG, "vah_validity: NULL context");
-    if (c->nstep < 1) return fail(VAH_ERR_STATE, "vah_validity: needs a completed step");
+    if (!c->stepped)
+        return fail(VAH_ERR_STATE, "vah_validity: needs a step completed since the last state upload");
     HIP_TRY(hipSetDevice(c->device));
     const long n = (long)c->lat.nx * c->lat.ny * c->lat.nz;
     // the stage-1 output set is free between steps: 4 interior-sized outputs fit in it
     double *out = c->qset[c->qs];
-    const double t = c->hydro.t0 + (double)(c->nstep - 1) * c->lat.dt;
+    // the reference's t after its step: t_n, as the driver computes it, plus dt
+    // (FullyDiscreteKurganovTadmorScheme.cpp:766,791)
+    const double t = (c->hydro.t0 + (double)(c->nstep - 1) * c->lat.dt) + c->lat.dt;
     hipLaunchKernelGGL(validity_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, c->qset[c->qo],
                        c->ebuf[c->ecur], c->pbuf[c->ecur], c->useq[c->uc], c->useq[c->up], c->fs, c->lat.nx,
                        c->lat.ny, c->lat.nz, (long)c->ncx, (long)c->ncx * c->ncy, t, c->lat.dt, c->hydro.etabar,

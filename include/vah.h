@@ -162,8 +162,10 @@ int vah_reduce(vah_ctx *ctx, vah_stats *out);
  * (knudsenNumberTaupiT / TaupiL / Taupi and Rpi of
  * old_cpu-vah_code/src/HydrodynamicValidity.cpp:195-247, evaluated with the
  * arguments of its call at FullyDiscreteKurganovTadmorScheme.cpp:791: the
- * step's start time and conserved variables, the new e, p, u).  Any output
- * may be NULL; valid until the next vah_step. */
+ * time after the step, t_n + dt, the conserved variables of the step's
+ * start, the new e, p, u).  Any output may be NULL.  Needs a step completed
+ * since the last vah_set_state / vah_set_primitives (VAH_ERR_STATE
+ * otherwise). */
 int vah_validity(vah_ctx *ctx, double *kn_T, double *kn_L, double *kn, double *r_pi);
 
 /* Per-launch HIP-event timing of the two stage kernels (off by default); the

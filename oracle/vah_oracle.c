@@ -1055,8 +1055,9 @@ double orc_time(const orc_ctx *c) { return c->t0 + (double)c->nstep * c->dt; }
 
 /* First-order validity diagnostics of checkValidityKernel
  * (HydrodynamicValidity.cpp:47-247) with the arguments the reference passes
- * after a step (FullyDiscreteKurganovTadmorScheme.cpp:791): the time t_n of
- * the step's start, the conserved variables of that start (q^n: after the
+ * after a step (FullyDiscreteKurganovTadmorScheme.cpp:791): the time
+ * t_{n+1} = t_n + dt (rungeKutta2 advances t at :766 before the call), the
+ * conserved variables of the step's start (q^n: after the
  * driver's swap they sit in Q), the new e, p, u and u^n as the previous
  * velocity.  Per interior cell:
  *   kn_T = 5 etabar |theta_T| / T,  kn_L = 5 etabar |zDzu| / T,
@@ -1065,7 +1066,7 @@ double orc_time(const orc_ctx *c) { return c->t0 + (double)c->nstep * c->dt; }
  * -1 before the first step.  Any output may be NULL. */
 int orc_validity(const orc_ctx *c, double *kn_T, double *kn_L, double *kn, double *r_pi) {
     if (c->nstep < 1) return -1;
-    const double t = c->t0 + (double)(c->nstep - 1) * c->dt;
+    const double t = (c->t0 + (double)(c->nstep - 1) * c->dt) + c->dt;
     const double dt = c->dt, etabar = c->etabar;
     const long n = (long)c->nx * c->ny * c->nz;
     const long sx = 1, sy = c->ncx, sz = (long)c->ncx * c->ncy;

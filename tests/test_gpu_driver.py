@@ -62,3 +62,26 @@ def test_cli_run_writes_reference_files(tmp_path, oracle_mod):
     assert np.max(np.abs(ebin - st["e"]) / st["e"]) < 1e-10
     edat = np.array([float(l.split("\t")[3]) for l in lines])
     assert np.max(np.abs(edat - st["e"])) <= 5.1e-9  # %.8f rounding
+
+
+def test_plugin_run_symbol_matches_cli(tmp_path):
+    """The reference's run() symbol, called by a program shaped like the
+    reference's Run.cpp with the reference's parameter structs
+    (tests/plugin_caller.cpp), writes the same files, byte for byte, as the
+    gpu-vah command line given the same properties."""
+    from tests.test_plugin_abi import build_caller
+    exe = build_caller(tmp_path)
+    conf, out_cli, out_plugin = tmp_path / "conf", tmp_path / "cli", tmp_path / "plugin"
+    for d in (conf, out_cli, out_plugin):
+        d.mkdir()
+    write_conf(conf, nx=21, ny=17, nt=20, ictype=2)
+    r = subprocess.run([EXE, "--config", str(conf), "-o", str(out_cli), "-h"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([exe, str(out_plugin), "21", "17", "20", "2"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0 and "hydro failed" not in r.stderr, r.stderr
+    assert r.stdout.count("Elapsed time/step") == 3
+    files = sorted(p.name for p in out_cli.iterdir())
+    assert len(files) == 17 * 3 and files == sorted(p.name for p in out_plugin.iterdir())
+    for name in files:
+        assert (out_cli / name).read_bytes() == (out_plugin / name).read_bytes(), name

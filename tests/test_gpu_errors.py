@@ -36,6 +36,15 @@ def test_calls_out_of_order():
     g.step(1)
     g.synchronize()
     assert np.all(np.isfinite(g.validity()["kn"]))
+    # a state uploaded at a later step has no step since: its q^n is unknown
+    st = g.state()
+    g.set_state(5, st["q"], st["e"], st["p"], st["u"], st["up"])
+    with pytest.raises(V.VahError) as exc:
+        g.validity()
+    assert exc.value.code == V.VAH_ERR_STATE
+    g.step(1)
+    g.synchronize()
+    assert np.all(np.isfinite(g.validity()["kn"]))
     g.close()
 
 

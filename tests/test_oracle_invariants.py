@@ -157,8 +157,9 @@ def test_validity_diagnostics_in_bjorken_flow(oracle_mod):
     """Bjorken flow (uniform lattice at rest): theta = 1/tau, the transverse
     expansion theta_T = 2 theta/3 + A vanishes (A = tau^2 sigma^etaeta =
     -2/(3 tau)) and zDzu = 1/tau, so Kn_T = 0 and Kn_L = Kn = 5 (eta/s)/(tau T).
-    The diagnostics use the start time of the last step and T of the new e
-    (the reference's call after the step).  pi_perp stays 0, so R_pi = 0."""
+    The diagnostics use the time after the last step, t_n + dt (rungeKutta2
+    advances t before its checkValidity call), and T of the new e.  pi_perp
+    stays 0, so R_pi = 0."""
     nx, ny, dt, t0 = 5, 4, 0.01, 0.5
     n = nx * ny
     e = np.full(n, E0)
@@ -168,7 +169,7 @@ def test_validity_diagnostics_in_bjorken_flow(oracle_mod):
         o.validity()
     o.step(7)
     v = o.validity()
-    tn = t0 + 6 * dt
+    tn = (t0 + 6 * dt) + dt
     T = T_single(o.state()["e"][0])
     assert np.all(np.abs(v["kn_T"]) < 1e-12)
     assert np.allclose(v["kn_L"], 5 * ETABAR / (tn * T), rtol=1e-12)

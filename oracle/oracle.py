@@ -58,6 +58,7 @@ def lib():
         L.orc_set_primitives.argtypes = [ctypes.c_void_p] + [_dp] * 7
         L.orc_set_state.argtypes = [ctypes.c_void_p, ctypes.c_long] + [_dp] * 5
         L.orc_step.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_set_pairing.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.orc_time.argtypes = [ctypes.c_void_p]
         L.orc_time.restype = ctypes.c_double
         L.orc_newton_fail.argtypes = [ctypes.c_void_p]
@@ -166,6 +167,12 @@ class Oracle:
         if not self.h:
             raise RuntimeError("oracle lattice already closed")
         return self.h
+
+    def set_pairing(self, mode):
+        """Test-only: 0 the reference's velocity pairing in the sources' time
+        derivative; 1 pairs u* with u^{n-1} in stage 2; 2 drops the stage-1
+        time derivative.  Modes 1 and 2 are planted mistakes."""
+        lib().orc_set_pairing(self._handle(), mode)
 
     def step(self, n=1):
         lib().orc_step(self._handle(), n)

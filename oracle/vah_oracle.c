@@ -97,6 +97,7 @@ typedef struct {
     double *u[4], *up[4], *uS[4]; /* ut, ux, uy, un */
     double *slab;
     long newton_fail; /* cell-recovery fallbacks in the last step */
+    int pairing;      /* 0: the reference's velocity pairs; 1, 2: planted mistakes for tests */
     long nan_hits;    /* cells where the reference would have exited on NaN */
     int nthreads;
 } orc_ctx;
@@ -878,13 +879,16 @@ static void swap_sets(double **a, double **b, int n) {
  * :791 writes only diagnostic arrays and is not restated. */
 static void rk2_step(orc_ctx *c, double t) {
     double dt = c->dt;
-    euler_stage(c, t, c->q, c->qS, c->u, c->up);
+    /* test-only planted mistakes (orc_set_pairing): mode 2 drops the stage-1
+     * time derivative (u^n paired with itself), mode 1 pairs u* with u^{n-1}
+     * in stage 2 */
+    euler_stage(c, t, c->q, c->qS, c->u, c->pairing == 2 ? c->u : c->up);
     t += dt;
     recover_all(c, t, c->qS, c->u, c->uS);
     regulate(c, t, c->qS, c->uS);
     ghost_fill(c, c->qS, c->uS);
 
-    euler_stage(c, t, c->qS, c->Q, c->uS, c->u);
+    euler_stage(c, t, c->qS, c->Q, c->uS, c->pairing == 1 ? c->up : c->u);
     convex_combination(c, c->q, c->Q);
     swap_sets(c->up, c->u, 4);
     recover_all(c, t, c->Q, c->up, c->u);
@@ -1052,6 +1056,11 @@ void orc_step(orc_ctx *c, int nsteps) {
 }
 
 double orc_time(const orc_ctx *c) { return c->t0 + (double)c->nstep * c->dt; }
+
+/* Test-only: plant a wrong velocity pairing in the time derivative of the
+ * source terms (see rk2_step), so that a test can show it detects one.
+ * 0 restores the reference's pairing. */
+void orc_set_pairing(orc_ctx *c, int mode) { c->pairing = mode; }
 
 /* First-order validity diagnostics of checkValidityKernel
  * (HydrodynamicValidity.cpp:47-247) with the arguments the reference passes

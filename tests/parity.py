@@ -73,3 +73,13 @@ def assert_parity(a, b, tau, tol=TOL_FP64, label=""):
     worst = max(errs.items(), key=lambda kv: kv[1])
     assert not bad, f"{label}: fields over {tol:g}: {bad}"
     return worst
+
+
+def table_T(tab, e):
+    """Temperature of the tabulated EOS (lne0, h, p, T) at energy density e:
+    linear in ln e between nodes (the device's table lookup)."""
+    lne0, h, _, T = tab
+    f = (np.log(e) - lne0) / h
+    i = int(np.clip(np.floor(f), 0, len(T) - 2))
+    w = f - i
+    return T[i] + w * (T[i + 1] - T[i])

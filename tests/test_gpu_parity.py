@@ -28,11 +28,13 @@ def run_pair(oracle_mod, nx, ny, nz, steps, e, u=None, pl=None, pi=None, dx=0.1,
 
 
 @pytest.mark.parametrize("flux_mode", [0, 1], ids=["shared", "per_cell"])
-def test_c1_shape_200_steps(oracle_mod, flux_mode):
-    """81x81x1, dx 0.1, tau0 0.5, dtau 0.01, 200 steps, conformal, eta/s 0.2, P_L = e/3."""
+def test_gaussian_81x81_200_steps(oracle_mod, flux_mode):
+    """Elliptic Gaussian profile on the 81x81x1 lattice of the optical-Glauber
+    event (dx 0.1, tau0 0.5, dtau 0.01, 200 steps, conformal, eta/s 0.2,
+    P_L = e/3); the optical-Glauber event itself is in test_gpu_workloads."""
     e = gaussian_ic(81, 81)
     a, b, tau = run_pair(oracle_mod, 81, 81, 1, 200, e, pl=e / 3, flux_mode=flux_mode)
-    worst = assert_parity(a, b, tau, label="81x81x1 optical Glauber")
+    worst = assert_parity(a, b, tau, label="81x81x1 Gaussian")
     print("worst field", worst)
 
 

@@ -1,0 +1,200 @@
+"""Pins of the oracle parts that no reference build reaches, by what the
+mathematics fixes (no formula of the oracle is retyped here):
+
+1. The regulator's signed ratios (FullyDiscreteKurganovTadmorScheme.cpp:597-665):
+   hand-built pi^{mu nu} whose rho follows in closed form -- a pure trace at
+   rest (trace and u.pi ratios equal +1, so rho = 1 whatever the size), its
+   mirror image (ratios -1: a signed maximum ignores them, an absolute value
+   would not), an eta-eta component whose z.pi ratio is 1/tau, and a
+   transverse traceless pi seen from a moving cell (u.pi ratio |u^x|/sqrt 2 on
+   one side of the flow only).
+
+2. The Heun composition with flow (rungeKutta2, :738-792): which velocity
+   pair feeds the source terms' time derivative in each stage.  A uniform
+   lattice with uniform longitudinal flow relative to Bjorken,
+   u = (cosh xi, 0, 0, sinh xi / tau), has no fluxes; its exact evolution
+   follows from energy-momentum conservation in Milne coordinates,
+       d T^tautau/dtau = -(T^tautau + tau^2 T^etaeta)/tau,
+       d T^taueta/dtau = -3 T^taueta / tau,
+   and the anisotropic P_L relaxation equation
+       u^tau dP_L/dtau = -(P_L - P)/tau_pi + (R~(P_L/e) e - 3 P_L) theta,
+       theta = d u^tau/dtau + u^tau/tau
+   (theta_T = 0, zDzu = theta, pi_perp = 0 for this flow).  Here theta
+   contains the time derivative of the flow that the scheme differences from
+   the stored velocities, so the pairing decides whether the lattice
+   converges to this solution.  The test also plants the two plausible
+   mistakes in the oracle and checks that they fail."""
+import numpy as np
+import pytest
+from scipy.integrate import solve_ivp
+
+from tests.test_oracle_invariants import E0, ETABAR, rtilde
+
+
+def p(a):
+    import ctypes
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+# ---------------------------------------------------------------- regulator
+def regulate(oracle_mod, t, q, e, pl_p, u):
+    """One cell through the oracle's regulator; returns the regulated q."""
+    q = np.array(q, dtype=float)
+    ee, pp = np.array([e]), np.array([pl_p])
+    oracle_mod.lib().orc_regulate_cell(t, p(q), p(ee), p(pp), p(np.asarray(u, dtype=float)))
+    return q
+
+
+def magnitude_ratio(oracle_mod, e, pl, pipi_abs):
+    """|pi| / sqrt(e^2 + P_L^2 + 2 P-hat_T^2) with P-hat_T = (e - P_L)/2 for e = 3p."""
+    pt = (e - pl) / 2
+    assert oracle_mod.lib().orc_pt_hat(e, e / 3, pl) == pytest.approx(pt, rel=1e-13)
+    return np.sqrt(pipi_abs) / np.sqrt(e * e + pl * pl + 2 * pt * pt)
+
+
+def fac(rho):
+    return 1.0 if rho < 1e-7 else np.tanh(rho) / rho
+
+
+E, PL = 10.0, 10.0 / 3
+
+
+@pytest.mark.parametrize("c", [1e-6, 0.3, 5.0, 60.0])
+def test_regulator_positive_trace_at_rest(oracle_mod, c):
+    """pi^tautau = c > 0 alone, cell at rest: g.pi = c and (u.pi)^tau = c, each
+    divided by |pi| = c gives +1, so rho = max(1, c/|T|) even for tiny c."""
+    t = 0.7
+    q = np.zeros(15)
+    q[4], q[5] = PL, c
+    out = regulate(oracle_mod, t, q, E, E / 3, [1, 0, 0, 0])
+    rho = max(1.0, magnitude_ratio(oracle_mod, E, PL, c * c))
+    assert out[5] == pytest.approx(c * fac(rho), rel=1e-14)
+    if c < 1:
+        assert out[5] == pytest.approx(c * np.tanh(1.0), rel=1e-14)
+
+
+@pytest.mark.parametrize("c", [1e-6, 0.3, 5.0, 60.0])
+def test_regulator_negative_trace_is_not_picked(oracle_mod, c):
+    """The mirror image pi^tautau = -c: both ratios are -1, which the signed
+    maximum passes over; rho is the magnitude ratio alone (an absolute value
+    would give rho >= 1 and shrink small pi by tanh 1)."""
+    t = 0.7
+    q = np.zeros(15)
+    q[4], q[5] = PL, -c
+    out = regulate(oracle_mod, t, q, E, E / 3, [1, 0, 0, 0])
+    rho = magnitude_ratio(oracle_mod, E, PL, c * c)
+    assert out[5] == pytest.approx(-c * fac(rho), rel=1e-14)
+    if c < 1e-3:
+        assert out[5] == -c  # rho < 1e-7: untouched
+
+
+@pytest.mark.parametrize("t", [0.5, 2.0])
+@pytest.mark.parametrize("sign", [1, -1])
+def test_regulator_eta_component_scales_with_tau(oracle_mod, t, sign):
+    """pi^etaeta = -sign c / tau^2 at rest: |pi| = c, g.pi = sign c (ratio
+    sign), (z.pi)^eta = z_eta pi^etaeta with z = (0,0,0,1/tau) gives
+    sign/tau.  So rho = max(1, 1/t) for sign = +1 (the tau^2 of the metric and
+    the 1/tau of z both enter), and the magnitude ratio for sign = -1."""
+    c = 1e-3
+    q = np.zeros(15)
+    q[4], q[14] = PL, -sign * c / t ** 2
+    out = regulate(oracle_mod, t, q, E, E / 3, [1, 0, 0, 0])
+    rho = max(1.0, 1.0 / t) if sign > 0 else magnitude_ratio(oracle_mod, E, PL, c * c)
+    assert out[14] == pytest.approx(q[14] * fac(rho), rel=1e-13)
+
+
+@pytest.mark.parametrize("ux", [0.8, -0.8])
+def test_regulator_u_ratio_with_flow(oracle_mod, ux):
+    """Traceless pi^xx = -pi^yy = m in a cell moving along x: |pi| = sqrt 2 m and
+    (u.pi)^x = u_x pi^xx = -u^x m, ratio -u^x/sqrt 2.  Flow towards -x makes
+    that ratio positive (rho = |u^x|/sqrt 2 when it exceeds the magnitude
+    ratio); flow towards +x leaves the magnitude ratio alone."""
+    m = 1e-4
+    ut = np.sqrt(1 + ux * ux)
+    q = np.zeros(15)
+    q[4], q[9], q[12] = PL, m, -m
+    out = regulate(oracle_mod, 1.0, q, E, E / 3, [ut, ux, 0, 0])
+    mag = magnitude_ratio(oracle_mod, E, PL, 2 * m * m)
+    rho = max(mag, -ux / np.sqrt(2))
+    assert out[9] == pytest.approx(m * fac(rho), rel=1e-13)
+    assert out[12] == pytest.approx(-m * fac(rho), rel=1e-13)
+    if ux < 0:
+        assert out[9] == pytest.approx(m * np.tanh(0.8 / np.sqrt(2)) / (0.8 / np.sqrt(2)), rel=1e-13)
+
+
+# ------------------------------------------------------- Heun composition
+def longitudinal_flow_rhs(t, y):
+    """Exact equations of uniform longitudinal flow in the conformal VAH
+    theory (see the module docstring); y = (T^tautau, T^taueta, P_L)."""
+    ttt, ttn, pl = y
+    A = t * ttn / (ttt + pl)            # tanh xi
+    xi = np.arctanh(A)
+    ch, sh = np.cosh(xi), np.sinh(xi)
+    e = (ttt + pl) / ch ** 2 - pl
+    T = (e / 13.8997) ** 0.25
+    dttt = -(ttt + e * sh * sh + pl * ch * ch) / t
+    dttn = -3 * ttn / t
+    # d xi/dtau = (al - be dP_L/dtau) / (1 - A^2), from A = tau T^taueta / (T^tautau + P_L)
+    al = (ttn + t * dttn) / (ttt + pl) - t * ttn * dttt / (ttt + pl) ** 2
+    be = t * ttn / (ttt + pl) ** 2
+    X = -(pl - e / 3) * T / (5 * ETABAR)
+    Y = rtilde(pl / e) * e - 3 * pl
+    k = 1 / (1 - A * A)
+    # u^tau dP_L/dtau = X + Y theta, theta = sinh xi xi' + cosh xi / tau: linear in dP_L/dtau
+    dpl = (X + Y * (sh * al * k + ch / t)) / (ch + Y * sh * be * k)
+    return [dttt, dttn, dpl]
+
+
+def lattice_run(oracle_mod, dt, t1, xi0, mode=0):
+    t0, n = 0.5, 4
+    o = oracle_mod.Oracle(2, 2, 1, 0.1, 0.1, 0.1, dt, ETABAR, t0, threads=1)
+    o.set_pairing(mode)
+    e = np.full(n, E0)
+    o.set_primitives(e, np.full(n, np.cosh(xi0)), np.zeros(n), np.zeros(n), np.full(n, np.sinh(xi0) / t0),
+                     pl=e / 3)
+    o.step(int(round((t1 - t0) / dt)))
+    st, tau = o.state(), o.time
+    o.close()
+    assert np.all(st["e"] == st["e"][0])  # uniform stays uniform
+    return st, tau
+
+
+def exact(tau, xi0):
+    t0 = 0.5
+    ch, sh = np.cosh(xi0), np.sinh(xi0)
+    y0 = [E0 * ch * ch + E0 / 3 * sh * sh, (E0 + E0 / 3) * ch * sh / t0, E0 / 3]
+    sol = solve_ivp(longitudinal_flow_rhs, (t0, tau), y0, method="DOP853", rtol=1e-12, atol=1e-12)
+    ttt, ttn, pl = sol.y[:, -1]
+    xi = np.arctanh(tau * ttn / (ttt + pl))
+    return (ttt + pl) / np.cosh(xi) ** 2 - pl, pl, xi
+
+
+def errors(oracle_mod, dt, mode=0, xi0=0.5, t1=1.5):
+    st, tau = lattice_run(oracle_mod, dt, t1, xi0, mode)
+    e, pl, xi = exact(tau, xi0)
+    return (abs(st["e"][0] / e - 1), abs(st["q"][4][0] / pl - 1), abs(np.arcsinh(tau * st["u"][3][0]) - xi), xi)
+
+
+def test_uniform_longitudinal_flow_converges_to_the_exact_solution(oracle_mod):
+    """The reference's pairing converges (first order: the scheme differences
+    the flow's time derivative backward in each stage) to the exact
+    solution, whose rapidity relaxes from 0.5 to ~0.24 by tau = 1.5 fm/c."""
+    errs = [errors(oracle_mod, dt) for dt in (0.01, 0.005, 0.0025)]
+    xi_end = errs[0][3]
+    assert 0.2 < xi_end < 0.3
+    for a, b in zip(errs, errs[1:]):
+        for k in range(2):  # e, P_L
+            assert 1.8 < a[k] / b[k] < 2.3
+    e_err, pl_err, xi_err, _ = errs[-1]
+    assert e_err < 3e-4 and pl_err < 2e-3 and xi_err < 2e-5
+
+
+@pytest.mark.parametrize("mode", [1, 2], ids=["stage2_u_star_with_u_prev", "stage1_without_time_derivative"])
+def test_planted_pairing_mistakes_fail_the_flow_pin(oracle_mod, mode):
+    """Planted in the oracle: u* paired with u^{n-1} in stage 2 (twice the
+    time derivative), or none in stage 1.  Both leave an O(1e-2) error in e
+    and P_L that does not shrink with dtau, which the pin above rejects."""
+    a = errors(oracle_mod, 0.005, mode)
+    b = errors(oracle_mod, 0.0025, mode)
+    assert a[0] > 1e-2 and b[0] > 1e-2
+    assert a[0] / b[0] < 1.2  # no convergence

@@ -1,5 +1,6 @@
 #!/usr/bin/env python3
-"""Throughput of the FP64 VAH Kurganov-Tadmor Heun-RK2 step on one MI355X.
+"""Throughput of the FP64 VAH Kurganov-Tadmor Heun-RK2 step on one MI355X,
+and of the batched and serving forms.
 
 Workload (default): one 3+1D event at real shape -- Monte-Carlo Glauber Pb+Pb
 at b = 6 fm (Woods-Saxon R = 6.62 fm, a = 0.546 fm, sigma_NN = 70 mb, wounded
@@ -8,9 +9,17 @@ half-width 1.5 with Gaussian tails 1.4, lattice 201 x 201 x 101,
 dx = deta = 0.1, tau0 = 0.5 fm/c, dtau = 0.02 fm/c, lattice-QCD-like EOS
 tabulated at 10^4 nodes, eta/s = 0.16.  A step is one Heun-RK2 step of the
 whole lattice (two fused stage launches); the state is resident in HBM.  The
-default 5 warmup + 50 timed steps stay in the part of the evolution where
-every cell's Newton recovery succeeds; longer runs restart the event every 60
-steps, outside the timed region (see the --steps help).
+value is the median over --reps repetitions of --steps timed steps.  The
+event's 0.5 fm hot spots break the scheme down at the dilute edge near
+step 80 (tau ~ 2.1 fm/c) at this dtau (the CPU oracle does the same), so a
+repetition restarts the event from its initial state every 80 steps, outside
+the timed region; the cost of each 10-step block of one evolution from
+tau = 0.5 to 2.1 fm/c is reported beside it.
+
+The same line carries a sample of the batched form (boost-invariant
+MC-Glauber events of uneven sizes in one batch, each run to freeze-out) and
+of the serving form (a Poisson request stream with 3+1D requests, continuous
+batching), both against the plain per-event FP64 path.
 
 Prints one JSON line.  --gpus above 1 is refused: the benchmark runs on one GPU.
 """
@@ -34,7 +43,7 @@ BYTES_PER_CELL_STAGE = (368, 488)  # algorithmic bytes per cell of the stage-1 /
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "stage_kernel_hbm_traffic_201x201x101_fp64.md")
 
 WORKLOADS = {
-    "mc3d": dict(nx=201, ny=201, nz=101, dx=0.1, dz=0.1, dt=0.02, t0=0.5, etabar=0.16, eos="table", ic="mc", restart=60,
+    "mc3d": dict(nx=201, ny=201, nz=101, dx=0.1, dz=0.1, dt=0.02, t0=0.5, etabar=0.16, eos="table", ic="mc", restart=80,
                name="3+1D MC-Glauber Pb+Pb b=6 fm, 201x201x101, dx=deta=0.1, tau0=0.5, dtau=0.02, "
                     "LQCD-like EOS table (1e4 nodes), eta/s=0.16"),
     "optical2d": dict(nx=81, ny=81, nz=1, dx=0.1, dz=0.1, dt=0.01, t0=0.5, etabar=0.2, eos="conformal", ic="optical",
@@ -241,6 +250,86 @@ def batch_errors(V, evs, tab, steps):
     for x in r:
         x["B"].close()
     return out
+
+
+def per_cell_errors(a, b):
+    """Per-cell relative errors of state a against FP64 state b: e, and u^mu
+    relative to u^tau (the bar of the batched and serving forms)."""
+    e = np.abs(a["e"] - b["e"]) / b["e"]
+    u = np.max(np.abs(a["u"] - b["u"]), axis=0) / b["u"][0]
+    return float(e.max()), float(u.max())
+
+
+def batch_sample(V, n_events, tab):
+    """The batched form on n_events events drawn like the batched workload
+    (boost-invariant MC-Glauber Pb+Pb, side U{101..261}, dx U[0.08,0.12],
+    dtau = 0.2 dx, b U[0,12] fm), each run to freeze-out (hottest cell below
+    150 MeV) in one FP64 batch; the same events through the plain per-event
+    FP64 path one after the other; and the split-precision batch, with its
+    per-cell error against the FP64 batch at the FP64 freeze-out steps."""
+    evs = batch_events(V, n_events, seed=2024)
+    cells = np.array([ev["nx"] * ev["ny"] for ev in evs])
+    r64 = run_batched(V, evs, tab, V.FP64, T_FREEZE_GEV, max_steps=[1000] * n_events)
+    nsteps = r64["stop_step"]
+    frozen = int(np.sum(np.array(r64["stop_state"]) == V.EVENT_FROZEN))
+    r32t = run_batched(V, evs, tab, V.FP32_SPLIT, T_FREEZE_GEV, max_steps=[1000] * n_events)
+    r32t["B"].close()
+    r32 = run_batched(V, evs, tab, V.FP32_SPLIT, 0.0, max_steps=nsteps)
+    errs = np.array([per_cell_errors(r32["B"].state(a, ("e", "u")), r64["B"].state(b, ("e", "u")))
+                     for a, b in zip(r32["slots"], r64["slots"])])
+    r32["B"].close()
+    r64["B"].close()
+    t_plain = sum(run_plain(V, ev, k, tab)[0] for ev, k in zip(evs, nsteps))
+    lat = np.array(r64["stop_time"])
+    cu = float(np.sum(cells * np.array(nsteps)))
+    return {
+        "workload": f"{n_events} boost-invariant MC-Glauber Pb+Pb events (1 fm nucleons), side U{{101..261}}, "
+                    "dx U[0.08,0.12], dtau=0.2dx, b U[0,12] fm, tau0=0.5, eta/s=0.16, LQCD-like EOS table, "
+                    "each to max T < 150 MeV, one batch",
+        "precision": "fp64", "events": n_events, "frozen": frozen,
+        "steps_min_max": [int(min(nsteps)), int(max(nsteps))],
+        "events_per_s": n_events / r64["wall"], "wall_s": r64["wall"], "cell_updates_per_s": cu / r64["wall"],
+        "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                      "note": "all events admitted at once; admission to the poll that saw the event frozen"},
+        "plain_fp64_events_per_s": n_events / t_plain,
+        "speedup_vs_plain_fp64": t_plain / r64["wall"],
+        "split_precision": {"events_per_s": n_events / r32t["wall"], "speedup_vs_plain_fp64": t_plain / r32t["wall"],
+                            "per_cell_max_rel_error_vs_fp64": {"e": float(errs[:, 0].max()),
+                                                               "u": float(errs[:, 1].max())},
+                            "events_within_1e-4": int(np.sum(np.max(errs, axis=1) <= 1e-4))},
+    }
+
+
+def serving_sample(V, n_req, rate, tab, budget_gb):
+    """The serving form on n_req requests of the serving mix (80 % boost-
+    invariant L^2, 20 % 3+1D L^2 x nz, per-request MC-Glauber seeds) arriving
+    as a Poisson stream: continuous batching over the FP64 batch engine,
+    against the sequential plain FP64 path on the same arrivals."""
+    serve = importlib.import_module("gpu-vah_amd.serve")
+    reqs, base = serving_requests(V, n_req, seed=77)
+    arrivals = base / rate
+    for r, a in zip(reqs, arrivals):
+        r.arrival = float(a)
+    B = V.Batch(budget_bytes=int(budget_gb * (1 << 30)), max_events=256, eos_table=tab, precision=V.FP64,
+                freezeout_T_GeV=T_FREEZE_GEV, steps_per_graph=16)
+    out = serve.serve(B, reqs, steps_per_poll=16)
+    B.close()
+    service = [run_plain(V, dict(nx=r.nx, ny=r.ny, nz=r.nz, dx=r.dx, dz=r.dz, dt=r.dt, t0=r.t0, etabar=r.etabar,
+                                 e=r.e), r.nstep, tab)[0] for r in reqs]
+    lat = serve.fifo_latencies(arrivals, service)
+    finish = max(a + l for a, l in zip(sorted(arrivals), lat))
+    return {
+        "workload": f"{n_req} Poisson requests at {rate:g}/s, 80% boost-invariant L^2 (L U{{121..301}}), "
+                    "20% 3+1D L^2 x nz (L U{121..201}, nz U{41..81}), dx=deta=0.1, dtau=0.02, MC-Glauber "
+                    "(1 fm nucleons) per-request seeds, retire at max T < 150 MeV",
+        "precision": "fp64", "requests": n_req, "requests_3d": int(sum(r.nz > 1 for r in reqs)),
+        "events_per_s": out["events_per_s"],
+        "latency_s": {"p50": out["p50_latency_s"], "p99": out["p99_latency_s"],
+                      "note": "arrival to retirement at freeze-out, queueing included"},
+        "peak_resident": out["peak_resident"], "hbm_budget_gib": budget_gb,
+        "sequential_plain_fp64": {"events_per_s": n_req / finish, "p50_latency_s": float(np.percentile(lat, 50)),
+                                  "p99_latency_s": float(np.percentile(lat, 99))},
+    }
 
 
 def bench_batch(V, args):
@@ -473,13 +562,18 @@ def calibrate_copy(mib):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50,
-                    help="timed steps.  5 + 50 steps of the 201x201x101 event cover tau = 0.5 .. 1.6 fm/c; with "
-                         "its 0.5 fm nucleon hot spots the scheme loses the Newton recovery at the dilute edge "
-                         "from step ~80 on (tau ~ 2.1 fm/c) and the event blows up near step 100, on the CPU "
-                         "oracle as on the GPU, so longer runs restart the event from its initial state every "
-                         "60 steps, outside the timed region")
+    ap.add_argument("--steps", type=int, default=100,
+                    help="timed steps per repetition.  With its 0.5 fm nucleon hot spots the 201x201x101 event "
+                         "loses the Newton recovery at the dilute edge from step ~80 on (tau ~ 2.1 fm/c) at "
+                         "dtau = 0.02 and blows up near step 100, on the CPU oracle as on the GPU, so the "
+                         "event restarts from its initial state every 80 steps, outside the timed region")
     ap.add_argument("--warmup", type=int, default=5, help="untimed steps before the timed ones")
+    ap.add_argument("--reps", type=int, default=5, help="repetitions of --steps timed steps; the median is reported")
+    ap.add_argument("--batch-events", type=int, default=64,
+                    help="default workload: events of the batched-form sample in the same line (0: none)")
+    ap.add_argument("--serve-requests", type=int, default=48,
+                    help="default workload: requests of the serving-form sample in the same line (0: none)")
+    ap.add_argument("--serve-rate", type=float, default=16.0, help="arrival rate [1/s] of the serving sample")
     ap.add_argument("--workload", choices=sorted(WORKLOADS) + ["batch", "serving", "precision"], default="mc3d",
                     help="mc3d (default): one 201x201x101 MC-Glauber event; optical2d: 81x81x1 optical Glauber; "
                          "batch: 256 boost-invariant events in one batch; serving: Poisson request stream; "
@@ -532,7 +626,7 @@ def main():
     # is restarted from its initial state every reset_steps steps, outside the
     # timed region: runs of any length time steps of the healthy evolution.
     # Other workloads run as one evolution.
-    reset_steps = w["restart"] or (args.warmup + args.steps + 1)
+    reset_steps = w["restart"] or (args.warmup + args.steps * args.reps + 1)
     age = [reset_steps]
 
     def run(k):
@@ -555,15 +649,27 @@ def main():
         return t
 
     run(args.warmup)
-    # timed region: K steps
+    # timed region: reps x K steps, the median repetition is the value
     g.profile(True)
-    elapsed = run(args.steps)
+    rep_ms = [run(args.steps) / args.steps * 1e3 for _ in range(args.reps)]
     prof = g.profile_read()
     g.profile(False)
+    ms = float(np.median(rep_ms))
+    # one evolution from the initial state, timed in 10-step blocks: the cost
+    # of the step as the flow develops (tau0 .. tau0 + reset_steps dtau)
+    evolution = None
+    if w["restart"]:
+        age[0] = reset_steps
+        blocks = [run(10) / 10 * 1e3 for _ in range(reset_steps // 10)]
+        st = g.stats()
+        evolution = {"ms_per_step_by_10_steps": [round(b, 4) for b in blocks],
+                     "tau_range": [w["t0"], w["t0"] + 10 * len(blocks) * w["dt"]],
+                     "recovery_fallbacks": st.recovery_fallbacks, "nan_cells": st.nan_cells,
+                     "max_vT_end": st.max_vT}
     st = g.stats()
     g.close()
 
-    value = n * args.steps / elapsed
+    value = n / (ms * 1e-3)
     # dominant kernel: the stage-2 launch (convex combination and the larger
     # read set); algorithmic bytes per launch over its mean launch duration
     stage_gbs = [BYTES_PER_CELL_STAGE[i] * n / (prof["avg_ms"][i] * 1e-3) / 1e9 for i in range(2)]
@@ -577,7 +683,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": elapsed / args.steps * 1e3,
+        "ms_per_step": ms,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -585,21 +691,33 @@ def main():
         "data": "synthetic",
         "config": {"workload": w["name"], "cells": n, "flux_mode": args.flux_mode,
                    "precision": args.precision, "restart_every_steps": w["restart"],
-                   "stage_kernel_ms": prof["avg_ms"], "recovery_fallbacks": st.recovery_fallbacks,
-                   "nan_cells": st.nan_cells},
+                   "reps": args.reps, "rep_ms_per_step": [round(x, 4) for x in rep_ms],
+                   "stage_kernel_ms": prof["avg_ms"], "stage_kernel_launches": prof["launches"],
+                   "recovery_fallbacks": st.recovery_fallbacks, "nan_cells": st.nan_cells,
+                   "evolution": evolution},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None if traffic is None else traffic[f"stage{dom + 1}"],
+                     "traffic_source": None if traffic is None else
+                     "rocprofv3 FETCH_SIZE + WRITE_SIZE of this kernel (copy-calibrated), read from "
+                     "profiles/stage_kernel_hbm_traffic_201x201x101_fp64.md; not measured in this run",
                      "kernel": f"fused KT stage-{dom + 1} kernel",
                      "algorithmic_bytes_per_launch": BYTES_PER_CELL_STAGE[dom] * n,
                      "avg_launch_ms": prof["avg_ms"][dom],
                      "per_stage_GBs": stage_gbs,
                      "step_GBs": BYTES_PER_CELL_UPDATE * value / 1e9,
+                     "step_frac": BYTES_PER_CELL_UPDATE * value / 1e9 / HBM_PEAK_GBS,
                      "traffic_per_stage": traffic},
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(w, e, eos_table, args.cpu_seconds)
+    if args.workload == "mc3d":
+        tab = eos_table if eos_table is not None else V.eos_lqcd_table(10000, 1e-8, 1e4)
+        if args.batch_events > 0:
+            line["batched"] = batch_sample(V, args.batch_events, tab)
+        if args.serve_requests > 0:
+            line["serving"] = serving_sample(V, args.serve_requests, args.serve_rate, tab, args.budget_gb)
     print(json.dumps(line))
 
 

@@ -178,85 +178,155 @@ bool lattice_ok(const vah_lattice *l) {
     return l && l->nx > 0 && l->ny > 0 && l->nz > 0 && l->dx > 0 && l->dy > 0 && l->dz > 0 && l->dt > 0;
 }
 
-StageParams make_params(const vah_ctx *c, int stage, double t) {
+// Launch geometry of one lattice: 16 x 15 tiles marching kz eta planes.
+struct Geometry {
+    int tiles_x, tiles_y, zchunks, kz, nblocks, grid;
+};
+
+Geometry make_geometry(const vah_lattice &lat, int eta_chunk) {
+    Geometry g;
+    g.tiles_x = (lat.nx + kTileX - 1) / kTileX;
+    g.tiles_y = (lat.ny + kTileY - 1) / kTileY;
+    // eta planes per workgroup: 3 by default (fastest in a sweep at 201x201x101)
+    const int kz_default = 3;
+    g.kz = lat.nz > 1 ? std::min(eta_chunk > 0 ? eta_chunk : kz_default, lat.nz) : 1;
+    g.zchunks = (lat.nz + g.kz - 1) / g.kz;
+    g.nblocks = g.tiles_x * g.tiles_y * g.zchunks;
+    g.grid = (g.nblocks + 7) / 8 * 8;
+    return g;
+}
+
+// Everything of a stage's parameters except its buffers and times.
+StageParams base_params(const vah_lattice &lat, const vah_hydro &hydro, const vah_host::Layout &lay,
+                        const Geometry &g, const Eos &eos, unsigned int *counters) {
     StageParams P;
-    const double *const_q[3] = {c->qset[0], c->qset[1], c->qset[2]};
-    P.fs = c->fs;
-    P.nx = c->lat.nx;
-    P.ny = c->lat.ny;
-    P.nz = c->lat.nz;
-    P.ncx = c->ncx;
-    P.ncy = c->ncy;
-    P.sy = c->ncx;
-    P.sz = (long)c->ncx * c->ncy;
-    P.tiles_x = c->tiles_x;
-    P.tiles_y = c->tiles_y;
-    P.zchunks = c->zchunks;
-    P.kz = c->kz;
-    P.nblocks = c->nblocks;
-    P.dt = c->lat.dt;
-    P.dt_inv = 1.0 / c->lat.dt;
-    P.etabar = c->hydro.etabar;
-    P.etabar_inv = 1.0 / c->hydro.etabar;
-    P.inv_dx = 1.0 / c->lat.dx;
-    P.inv_dy = 1.0 / c->lat.dy;
-    P.inv_dz = 1.0 / c->lat.dz;
-    P.hx = 1 / c->lat.dx / 2;
-    P.hy = 1 / c->lat.dy / 2;
-    P.hz = 1 / c->lat.dz / 2;
-    P.eos = c->eos;
-    P.counters = c->counters;
+    P.fs = lay.fs;
+    P.nx = lat.nx;
+    P.ny = lat.ny;
+    P.nz = lat.nz;
+    P.ncx = lay.ncx;
+    P.ncy = lay.ncy;
+    P.sy = lay.ncx;
+    P.sz = (long)lay.ncx * lay.ncy;
+    P.tiles_x = g.tiles_x;
+    P.tiles_y = g.tiles_y;
+    P.zchunks = g.zchunks;
+    P.kz = g.kz;
+    P.nblocks = g.nblocks;
+    P.dt = lat.dt;
+    P.dt_inv = 1.0 / lat.dt;
+    P.etabar = hydro.etabar;
+    P.etabar_inv = 1.0 / hydro.etabar;
+    P.inv_dx = 1.0 / lat.dx;
+    P.inv_dy = 1.0 / lat.dy;
+    P.inv_dz = 1.0 / lat.dz;
+    P.hx = 1 / lat.dx / 2;
+    P.hy = 1 / lat.dy / 2;
+    P.hz = 1 / lat.dz / 2;
+    P.eos = eos;
+    P.counters = counters;
     P.emax = nullptr;
-    const int e_in = stage == 1 ? c->ecur : 1 - c->ecur;
-    P.e_in = c->ebuf[e_in];
-    P.p_in = c->pbuf[e_in];
-    P.e_out = c->ebuf[1 - e_in];
-    P.p_out = c->pbuf[1 - e_in];
-    if (stage == 1) {
-        P.qin = const_q[c->qi];
-        P.qout = c->qset[c->qs];
-        P.qn = const_q[c->qi];
-        P.ucur = c->useq[c->uc];
-        P.uprev = c->useq[c->up];
-        P.uout = c->useq[c->us];
-        P.t_src = t;
-        P.t_rec = t + c->lat.dt;
-    } else {
-        P.qin = const_q[c->qs];
-        P.qout = c->qset[c->qo];
-        P.qn = const_q[c->qi];
-        P.ucur = c->useq[c->us];
-        P.uprev = c->useq[c->uc];
-        P.uout = c->useq[c->up];
-        P.t_src = t + c->lat.dt;
-        P.t_rec = t + c->lat.dt;
-    }
     return P;
 }
 
+// The buffers of one Heun step (rungeKutta2, FullyDiscreteKurganovTadmorScheme.cpp:761-789):
+// stage 1 reads q^n, e/p^n, u^n, u^{n-1} and writes q*, e/p*, u*; stage 2 reads
+// q*, e/p*, u*, u^n and q^n (convex combination) and writes q^{n+1}, e/p^{n+1}
+// and u^{n+1}.
+struct StepBuffers {
+    const double *q_n;
+    double *q_star, *q_next;
+    const double *e_n, *p_n;
+    double *e_star, *p_star, *e_next, *p_next;
+    const double *u_n, *u_prev;
+    double *u_star, *u_next;
+};
+
+void set_stage_buffers(StageParams &P, const StepBuffers &B, int stage, double t, double dt) {
+    if (stage == 1) {
+        P.qin = B.q_n;
+        P.qout = B.q_star;
+        P.qn = B.q_n;
+        P.e_in = B.e_n;
+        P.p_in = B.p_n;
+        P.e_out = B.e_star;
+        P.p_out = B.p_star;
+        P.ucur = B.u_n;
+        P.uprev = B.u_prev;
+        P.uout = B.u_star;
+        P.t_src = t;
+        P.t_rec = t + dt;
+    } else {
+        P.qin = B.q_star;
+        P.qout = B.q_next;
+        P.qn = B.q_n;
+        P.e_in = B.e_star;
+        P.p_in = B.p_star;
+        P.e_out = B.e_next;
+        P.p_out = B.p_next;
+        P.ucur = B.u_star;
+        P.uprev = B.u_n;
+        P.uout = B.u_next;
+        P.t_src = t + dt;
+        P.t_rec = t + dt;
+    }
+}
+
+StageParams make_params(const vah_ctx *c, int stage, double t) {
+    Geometry g{c->tiles_x, c->tiles_y, c->zchunks, c->kz, c->nblocks, c->grid};
+    StageParams P = base_params(c->lat, c->hydro, c->lay, g, c->eos, c->counters);
+    StepBuffers B;
+    B.q_n = c->qset[c->qi];
+    B.q_star = c->qset[c->qs];
+    B.q_next = c->qset[c->qo];
+    B.e_n = c->ebuf[c->ecur];
+    B.p_n = c->pbuf[c->ecur];
+    B.e_star = c->ebuf[1 - c->ecur];
+    B.p_star = c->pbuf[1 - c->ecur];
+    B.e_next = c->ebuf[c->ecur];
+    B.p_next = c->pbuf[c->ecur];
+    B.u_n = c->useq[c->uc];
+    B.u_prev = c->useq[c->up];
+    B.u_star = c->useq[c->us];
+    B.u_next = c->useq[c->up];
+    set_stage_buffers(P, B, stage, t, c->lat.dt);
+    return P;
+}
+
+// what a stage launch needs besides its parameters
+struct LaunchCfg {
+    int grid;
+    hipStream_t stream;
+    int precision, flux_mode;
+    bool d3;
+};
+
+LaunchCfg launch_cfg(const vah_ctx *c) {
+    return LaunchCfg{c->grid, c->stream, c->opt.precision, c->opt.flux_mode, c->lat.nz > 1};
+}
+
 template <bool D3, bool S2, bool SH, typename R>
-void launch(const vah_ctx *c, const StageParams &P) {
-    hipLaunchKernelGGL((stage_kernel<D3, S2, SH, R>), dim3(c->grid), dim3(NTHREADS), 0, c->stream, P);
+void launch(const LaunchCfg &L, const StageParams &P) {
+    hipLaunchKernelGGL((stage_kernel<D3, S2, SH, R>), dim3(L.grid), dim3(NTHREADS), 0, L.stream, P);
 }
 
 template <bool D3, bool S2>
-void launch_variant(const vah_ctx *c, const StageParams &P) {
-    if (c->opt.precision == VAH_FP32_SPLIT)
-        launch<D3, S2, true, float>(c, P);
-    else if (c->opt.flux_mode == VAH_FLUX_SHARED)
-        launch<D3, S2, true, double>(c, P);
+void launch_variant(const LaunchCfg &L, const StageParams &P) {
+    if (L.precision == VAH_FP32_SPLIT)
+        launch<D3, S2, true, float>(L, P);
+    else if (L.flux_mode == VAH_FLUX_SHARED)
+        launch<D3, S2, true, double>(L, P);
     else
-        launch<D3, S2, false, double>(c, P);
+        launch<D3, S2, false, double>(L, P);
 }
 
-void launch_stage(const vah_ctx *c, int stage, const StageParams &P) {
-    const bool d3 = c->lat.nz > 1;
-    if (d3) {
-        if (stage == 1) launch_variant<true, false>(c, P);
-        else launch_variant<true, true>(c, P);
+void launch_stage(const LaunchCfg &L, int stage, const StageParams &P) {
+    if (L.d3) {
+        if (stage == 1) launch_variant<true, false>(L, P);
+        else launch_variant<true, true>(L, P);
     } else {
-        if (stage == 1) launch_variant<false, false>(c, P);
-        else launch_variant<false, true>(c, P);
+        if (stage == 1) launch_variant<false, false>(L, P);
+        else launch_variant<false, true>(L, P);
     }
 }
 
@@ -491,16 +561,15 @@ int vah_create(const vah_lattice *lat, const vah_hydro *hydro, const vah_options
     }
 
     // launch geometry
-    c->tiles_x = (lat->nx + kTileX - 1) / kTileX;
-    c->tiles_y = (lat->ny + kTileY - 1) / kTileY;
-    // eta planes per workgroup: 4 in double precision; 8 in the split form, whose
-    // eta-face plane ring amortises its fill over a longer march (both measured
-    // fastest at 201x201x101)
-    const int kz_default = 3;
-    c->kz = lat->nz > 1 ? std::min(opt.eta_chunk > 0 ? opt.eta_chunk : kz_default, lat->nz) : 1;
-    c->zchunks = (lat->nz + c->kz - 1) / c->kz;
-    c->nblocks = c->tiles_x * c->tiles_y * c->zchunks;
-    c->grid = (c->nblocks + 7) / 8 * 8;
+    {
+        Geometry g = make_geometry(*lat, opt.eta_chunk);
+        c->tiles_x = g.tiles_x;
+        c->tiles_y = g.tiles_y;
+        c->kz = g.kz;
+        c->zchunks = g.zchunks;
+        c->nblocks = g.nblocks;
+        c->grid = g.grid;
+    }
     c->red_blocks = 1024;
     if (hipMalloc(&c->red_buf, sizeof(double) * 4 * c->red_blocks) != hipSuccess)
         return cleanup(fail(VAH_ERR_NOMEM, "vah_create: reduction buffer"));
@@ -548,7 +617,10 @@ int vah_set_state(vah_ctx *c, long long nstep, const double *q, const double *e,
     if ((rc = upload_fields(c, c->useq[c->uc], 4, u, buf)) != VAH_OK) return rc;
     if ((rc = upload_fields(c, c->useq[c->up], 4, up, buf)) != VAH_OK) return rc;
     if ((rc = fill_ghosts(c)) != VAH_OK) return rc;
+    // NaN and fallback counts restart with the new state
+    HIP_TRY(hipMemsetAsync(c->counters, 0, 4 * sizeof(unsigned), c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
+    c->counters_seen[0] = c->counters_seen[1] = 0;
     c->nstep = nstep;
     c->have_state = true;
     c->stepped = false;
@@ -570,13 +642,13 @@ int vah_step(vah_ctx *c, int nsteps) {
                 HIP_TRY(hipEventCreate(&a));
                 HIP_TRY(hipEventCreate(&b));
                 HIP_TRY(hipEventRecord(a, c->stream));
-                launch_stage(c, stage, P);
+                launch_stage(launch_cfg(c), stage, P);
                 HIP_TRY(hipEventRecord(b, c->stream));
                 c->ev.push_back(a);
                 c->ev.push_back(b);
                 c->ev_kind.push_back(stage - 1);
             } else {
-                launch_stage(c, stage, P);
+                launch_stage(launch_cfg(c), stage, P);
             }
             HIP_TRY(hipGetLastError());
         }

@@ -95,8 +95,8 @@ typedef struct vah_stats {
     double max_e, max_T;       /* over interior cells [fm^-4], [fm^-1] */
     double total_energy;       /* tau * sum(T^tautau) dx dy deta  [fm^-3 * fm^3 ...] */
     double max_vT;             /* max transverse speed |v_T| */
-    long long recovery_fallbacks; /* cells where the Newton recovery fell back (last step) */
-    long long nan_cells;       /* cells that produced NaN (last step) */
+    long long recovery_fallbacks; /* cell recoveries that fell back, since the last state upload */
+    long long nan_cells;       /* cell updates that produced NaN, since the last state upload */
 } vah_stats;
 
 typedef struct vah_profile {

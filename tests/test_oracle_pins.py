@@ -198,3 +198,80 @@ def test_planted_pairing_mistakes_fail_the_flow_pin(oracle_mod, mode):
     b = errors(oracle_mod, 0.0025, mode)
     assert a[0] > 1e-2 and b[0] > 1e-2
     assert a[0] / b[0] < 1.2  # no convergence
+
+
+# ------------------------------------------------------ validity diagnostics
+def test_validity_on_uniform_longitudinal_flow(oracle_mod):
+    """checkValidity's Knudsen numbers (HydrodynamicValidity.cpp:195-247) with
+    u^eta != 0: for uniform longitudinal flow theta = cosh xi / tau +
+    sinh xi dxi/dtau, the transverse expansion theta_T vanishes and
+    zDzu = theta.  The maps converge (first order, like the lattice's time
+    derivative of u) to 5 (eta/s) theta / T from the exact solution, kn_T to 0."""
+    rows = []
+    for dt in (0.01, 0.005, 0.0025):
+        t0, n = 0.5, 4
+        o = oracle_mod.Oracle(2, 2, 1, 0.1, 0.1, 0.1, dt, ETABAR, t0, threads=1)
+        e = np.full(n, E0)
+        o.set_primitives(e, np.full(n, np.cosh(0.5)), np.zeros(n), np.zeros(n), np.full(n, np.sinh(0.5) / t0),
+                         pl=e / 3)
+        o.step(int(round(0.5 / dt)))
+        v = o.validity()
+        T = (o.state()["e"][0] / 13.8997) ** 0.25
+        tau = o.time
+        o.close()
+        h = 1e-5
+        _, _, xi = exact(tau, 0.5)
+        dxi = (exact(tau + h, 0.5)[2] - exact(tau - h, 0.5)[2]) / (2 * h)
+        kn_exact = 5 * ETABAR * (np.cosh(xi) / tau + np.sinh(xi) * dxi) / T
+        rows.append((abs(v["kn"][0] / kn_exact - 1), v["kn_T"][0] / v["kn"][0], abs(v["kn_L"][0] / v["kn"][0] - 1)))
+    for a, b in zip(rows, rows[1:]):
+        for k in range(3):
+            assert 1.8 < a[k] / b[k] < 2.3
+    assert rows[-1][0] < 3e-4 and rows[-1][1] < 1e-3 and rows[-1][2] < 1e-3
+
+
+def test_validity_longitudinal_rate_without_eta_flow(oracle_mod):
+    """Boost-invariant transverse flow (u^eta = 0): the longitudinal expansion
+    rate is exactly u^tau / tau (z = (0, 0, 0, 1/tau)), so
+    kn_L = 5 (eta/s) u^tau / (tau T) cell by cell, and kn = kn_T + kn_L where
+    theta_T >= 0 (an expanding fireball)."""
+    from tests.parity import gaussian_ic, rest_frame
+    nx, ny = 24, 20
+    e = gaussian_ic(nx, ny, sx=1.0, sy=1.4, peak=60.0)
+    o = oracle_mod.Oracle(nx, ny, 1, 0.1, 0.1, 0.1, 0.01, ETABAR, 0.5, threads=2)
+    o.set_primitives(e, *rest_frame(nx * ny), pl=e / 3)
+    o.step(40)
+    v, st, tau = o.validity(), o.state(), o.time
+    o.close()
+    assert np.max(np.abs(st["u"][1])) > 0.05
+    T = (st["e"] / 13.8997) ** 0.25
+    np.testing.assert_allclose(v["kn_L"], 5 * ETABAR * st["u"][0] / (tau * T), rtol=1e-6)
+
+
+def test_inverse_reynolds_number_by_tensor_contraction(oracle_mod):
+    """R_pi = sqrt|pi^{mu nu} pi_{mu nu}| / |P-hat_T| on a flowing 3+1D state
+    with pi^{mu nu} != 0 and u^eta != 0: the contraction is done here as a
+    generic einsum with the Milne metric diag(1, -1, -1, -tau^2) at the time
+    after the step, pi and P_L from the step's start (its conserved
+    variables), P-hat_T = (e - P_L)/2 for the conformal EOS with the new e."""
+    from tests.parity import gaussian_ic, rest_frame
+    nx, ny, nz = 12, 10, 6
+    e = gaussian_ic(nx, ny, nz, dz=0.4, sx=0.8, sy=1.1, tilt=0.3, peak=60.0, seed=3)
+    n = nx * ny * nz
+    o = oracle_mod.Oracle(nx, ny, nz, 0.1, 0.1, 0.4, 0.01, ETABAR, 0.5, threads=2)
+    o.set_primitives(e, *rest_frame(n), pl=e / 3)
+    o.step(15)
+    q0 = o.state()["q"]
+    o.step(1)
+    v, st, tau = o.validity(), o.state(), o.time
+    o.close()
+    assert np.max(np.abs(st["u"][3])) > 1e-3 and np.max(np.abs(q0[8])) > 0  # u^eta and pi^taueta present
+    pi = np.zeros((n, 4, 4))
+    comp = {(0, 0): 5, (0, 1): 6, (0, 2): 7, (0, 3): 8, (1, 1): 9, (1, 2): 10, (1, 3): 11, (2, 2): 12, (2, 3): 13,
+            (3, 3): 14}
+    for (a, b), f in comp.items():
+        pi[:, a, b] = pi[:, b, a] = q0[f]
+    g = np.diag([1.0, -1.0, -1.0, -tau * tau])
+    pipi = np.einsum("nab,ac,bd,ncd->n", pi, g, g, pi)
+    r = np.sqrt(np.abs(pipi)) / np.abs((st["e"] - q0[4]) / 2)
+    np.testing.assert_allclose(v["r_pi"], r, rtol=1e-12)

@@ -231,27 +231,6 @@ def run_plain(V, ev, steps, tab):
     return time.perf_counter() - t0, clean
 
 
-def split_errors(a, b):
-    """Errors of a split-precision state a against the FP64 state b: per-cell
-    relative error of e and of u^mu (relative to u^tau), max and RMS over all
-    cells, and the largest |delta e| relative to the event's largest e."""
-    e = np.abs(a["e"] - b["e"]) / b["e"]
-    u = np.max(np.abs(a["u"] - b["u"]), axis=0) / b["u"][0]
-    return (float(e.max()), float(np.sqrt(np.mean(e * e))), float(u.max()), float(np.sqrt(np.mean(u * u))),
-            float(np.max(np.abs(a["e"] - b["e"])) / b["e"].max()))
-
-
-def batch_errors(V, evs, tab, steps):
-    """Split precision against FP64 for every event after the same number of
-    steps (freeze-out off): the per-event error tuples of split_errors."""
-    r = [run_batched(V, evs, tab, prec, 0.0, max_steps=[steps] * len(evs)) for prec in (V.FP32_SPLIT, V.FP64)]
-    out = np.array([split_errors(r[0]["B"].state(a, ("e", "u")), r[1]["B"].state(b, ("e", "u")))
-                    for a, b in zip(r[0]["slots"], r[1]["slots"])])
-    for x in r:
-        x["B"].close()
-    return out
-
-
 def per_cell_errors(a, b):
     """Per-cell relative errors of state a against FP64 state b: e, and u^mu
     relative to u^tau (the bar of the batched and serving forms)."""
@@ -260,7 +239,7 @@ def per_cell_errors(a, b):
     return float(e.max()), float(u.max())
 
 
-def batch_sample(V, n_events, tab):
+def batch_sample(V, n_events, tab, autotune_steps=0, max_steps=1000):
     """The batched form on n_events events drawn like the batched workload
     (boost-invariant MC-Glauber Pb+Pb, side U{101..261}, dx U[0.08,0.12],
     dtau = 0.2 dx, b U[0,12] fm), each run to freeze-out (hottest cell below
@@ -269,10 +248,12 @@ def batch_sample(V, n_events, tab):
     per-cell error against the FP64 batch at the FP64 freeze-out steps."""
     evs = batch_events(V, n_events, seed=2024)
     cells = np.array([ev["nx"] * ev["ny"] for ev in evs])
-    r64 = run_batched(V, evs, tab, V.FP64, T_FREEZE_GEV, max_steps=[1000] * n_events)
+    r64 = run_batched(V, evs, tab, V.FP64, T_FREEZE_GEV, max_steps=[max_steps] * n_events,
+                      autotune_steps=autotune_steps)
     nsteps = r64["stop_step"]
     frozen = int(np.sum(np.array(r64["stop_state"]) == V.EVENT_FROZEN))
-    r32t = run_batched(V, evs, tab, V.FP32_SPLIT, T_FREEZE_GEV, max_steps=[1000] * n_events)
+    r32t = run_batched(V, evs, tab, V.FP32_SPLIT, T_FREEZE_GEV, max_steps=[max_steps] * n_events,
+                       autotune_steps=autotune_steps)
     r32t["B"].close()
     r32 = run_batched(V, evs, tab, V.FP32_SPLIT, 0.0, max_steps=nsteps)
     errs = np.array([per_cell_errors(r32["B"].state(a, ("e", "u")), r64["B"].state(b, ("e", "u")))
@@ -291,6 +272,8 @@ def batch_sample(V, n_events, tab):
         "events_per_s": n_events / r64["wall"], "wall_s": r64["wall"], "cell_updates_per_s": cu / r64["wall"],
         "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                       "note": "all events admitted at once; admission to the poll that saw the event frozen"},
+        "autotune": r64["autotune"],
+        "steps_percentiles": {str(q): float(np.percentile(nsteps, q)) for q in (0, 50, 90, 99, 100)},
         "plain_fp64_events_per_s": n_events / t_plain,
         "speedup_vs_plain_fp64": t_plain / r64["wall"],
         "split_precision": {"events_per_s": n_events / r32t["wall"], "speedup_vs_plain_fp64": t_plain / r32t["wall"],
@@ -300,13 +283,24 @@ def batch_sample(V, n_events, tab):
     }
 
 
+def plain_service(V, r, steps, tab):
+    """Service time of one request on the sequential plain FP64 path: its
+    initial state generated on the host (vah_ic_mc_glauber), then one lattice
+    context from set-up to teardown for the same number of steps."""
+    t0 = time.perf_counter()
+    e, _ = V.ic_mc_glauber(r.nx, r.ny, r.nz, r.dx, r.dz, r.glauber)
+    t_ic = time.perf_counter() - t0
+    return t_ic + run_plain(V, dict(nx=r.nx, ny=r.ny, nz=r.nz, dx=r.dx, dz=r.dz, dt=r.dt, t0=r.t0, etabar=r.etabar,
+                                    e=e), steps, tab)[0]
+
+
 def serving_sample(V, n_req, rate, tab, budget_gb):
     """The serving form on n_req requests of the serving mix (80 % boost-
     invariant L^2, 20 % 3+1D L^2 x nz, per-request MC-Glauber seeds) arriving
     as a Poisson stream: continuous batching over the FP64 batch engine,
     against the sequential plain FP64 path on the same arrivals."""
     serve = importlib.import_module("gpu-vah_amd.serve")
-    reqs, base = serving_requests(V, n_req, seed=77)
+    reqs, base = serving_requests(V, n_req, seed=77, device_ic=True)
     arrivals = base / rate
     for r, a in zip(reqs, arrivals):
         r.arrival = float(a)
@@ -314,14 +308,14 @@ def serving_sample(V, n_req, rate, tab, budget_gb):
                 freezeout_T_GeV=T_FREEZE_GEV, steps_per_graph=16)
     out = serve.serve(B, reqs, steps_per_poll=16)
     B.close()
-    service = [run_plain(V, dict(nx=r.nx, ny=r.ny, nz=r.nz, dx=r.dx, dz=r.dz, dt=r.dt, t0=r.t0, etabar=r.etabar,
-                                 e=r.e), r.nstep, tab)[0] for r in reqs]
+    service = [plain_service(V, r, r.nstep, tab) for r in reqs]
     lat = serve.fifo_latencies(arrivals, service)
     finish = max(a + l for a, l in zip(sorted(arrivals), lat))
     return {
         "workload": f"{n_req} Poisson requests at {rate:g}/s, 80% boost-invariant L^2 (L U{{121..301}}), "
                     "20% 3+1D L^2 x nz (L U{121..201}, nz U{41..81}), dx=deta=0.1, dtau=0.02, MC-Glauber "
-                    "(1 fm nucleons) per-request seeds, retire at max T < 150 MeV",
+                    "(1 fm nucleons) per-request seeds generated on the device at admission, retire at "
+                    "max T < 150 MeV; C++ request loop (vah_serve)",
         "precision": "fp64", "requests": n_req, "requests_3d": int(sum(r.nz > 1 for r in reqs)),
         "events_per_s": out["events_per_s"],
         "latency_s": {"p50": out["p50_latency_s"], "p99": out["p99_latency_s"],
@@ -333,87 +327,31 @@ def serving_sample(V, n_req, rate, tab, budget_gb):
 
 
 def bench_batch(V, args):
+    """The batched form alone on args.events events (one JSON line)."""
     tab = V.eos_lqcd_table(10000, 1e-8, 1e4)
-    evs = batch_events(V, args.events, seed=2024)
-    cells = np.array([ev["nx"] * ev["ny"] for ev in evs])
-    # 1. the batched split-precision form, freeze-out at T < 150 MeV (autotuned)
-    r32 = run_batched(V, evs, tab, V.FP32_SPLIT, T_FREEZE_GEV, autotune_steps=args.autotune_steps,
-                      max_steps=[args.max_steps] * len(evs))
-    nsteps = r32["stop_step"]
-    states = np.array(r32["stop_state"])
-    for i in np.nonzero(states != V.EVENT_FROZEN)[0]:
-        ev = evs[i]
-        print(f"event {i}: L={ev['nx']} dx={ev['dx']:.3f} b={ev['b']:.1f} stopped with state {states[i]} "
-              f"at step {nsteps[i]}", file=sys.stderr, flush=True)
-    # 2. the batched FP64 form for the same step counts (freeze-out off)
-    r64 = run_batched(V, evs, tab, V.FP64, 0.0, max_steps=nsteps)
-    errs = np.array([split_errors(r32["B"].state(s32, ("e", "u")), r64["B"].state(s64, ("e", "u")))
-                     for s32, s64 in zip(r32["slots"], r64["slots"])])
-    frozen = states == V.EVENT_FROZEN
-    ef = errs[frozen] if frozen.any() else errs
-    e100 = batch_errors(V, evs, tab, 100)
-    r32["B"].close()
-    r64["B"].close()
-    # 3. the plain per-event FP64 path, one event after the other, same step counts
-    plain = [run_plain(V, ev, k, tab) for ev, k in zip(evs, nsteps)]
-    t_plain = [p[0] for p in plain]
-    cell_updates = float(np.sum(cells * np.array(nsteps)))
-    n = len(evs)
-    lat = np.array(r32["stop_time"])
+    r = batch_sample(V, args.events, tab, autotune_steps=args.autotune_steps, max_steps=args.max_steps)
     line = {
-        "metric": "events/s",
-        "value": n / r32["wall"],
-        "unit": "events/s",
-        "n_gpus": 1,
-        "steps": int(np.max(nsteps)),
-        "warmup": 0,
-        "ms_per_step": None,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32/f64",
-        "data": "synthetic",
-        "config": {"workload": f"{n} boost-invariant MC-Glauber Pb+Pb events (1 fm nucleon width), side U{{101..261}}, "
-                               "dx U[0.08,0.12], "
-                               "dtau=0.2dx, b U[0,12] fm, tau0=0.5, eta/s=0.16, LQCD-like EOS table, "
-                               "each run to max T < 150 MeV, one batch, split precision",
-                   "events": n, "cells_total": int(cells.sum()), "steps_min_max": [int(min(nsteps)), int(max(nsteps))],
-                   "cell_updates": cell_updates, "autotune": r32["autotune"],
-                   "stopped_frozen": int(np.sum(states == V.EVENT_FROZEN)),
-                   "stopped_nan": int(np.sum(states == V.EVENT_NAN)),
-                   "stopped_step_limit": int(np.sum(states == V.EVENT_STEP_LIMIT)),
-                   "plain_fp64_runs_with_nan": int(sum(not p[1] for p in plain))},
-        "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
-                      "note": "all events admitted at t=0; latency = time to the poll that saw it frozen"},
-        "batched_fp32_split": {"events_per_s": n / r32["wall"], "wall_s": r32["wall"], "admission_s": r32["admit_s"],
-                               "cell_updates_per_s": cell_updates / r32["wall"]},
-        "batched_fp64": {"events_per_s": n / r64["wall"], "wall_s": r64["wall"],
-                         "cell_updates_per_s": cell_updates / r64["wall"]},
-        "plain_fp64_per_event": {"events_per_s": n / sum(t_plain), "wall_s": sum(t_plain),
-                                 "cell_updates_per_s": cell_updates / sum(t_plain)},
-        "speedup_vs_plain_fp64": sum(t_plain) / r32["wall"],
-        "split_vs_fp64_error": {
-            "step_100_all_events": {"e_max": float(e100[:, 0].max()), "e_rms": float(np.sqrt(np.mean(e100[:, 1] ** 2))),
-                                    "u_max": float(e100[:, 2].max()), "u_rms": float(np.sqrt(np.mean(e100[:, 3] ** 2))),
-                                    "de_over_max_e": float(e100[:, 4].max())},
-            "at_freeze_out": {"events": int(len(ef)), "e_max": float(ef[:, 0].max()),
-                              "e_rms": float(np.sqrt(np.mean(ef[:, 1] ** 2))), "u_max": float(ef[:, 2].max()),
-                              "u_rms": float(np.sqrt(np.mean(ef[:, 3] ** 2))), "de_over_max_e": float(ef[:, 4].max())},
-            "note": "per-cell relative errors; de_over_max_e = max |delta e| / the event's max e"},
-        "steps_percentiles": {str(q): float(np.percentile(nsteps, q)) for q in (0, 50, 90, 99, 100)},
-        "roofline": {"bound": "hbm", "achieved": BYTES_PER_CELL_UPDATE * cell_updates / r32["wall"] / 1e9,
+        "metric": "events/s", "value": r["events_per_s"], "unit": "events/s", "n_gpus": 1,
+        "steps": r["steps_min_max"][1], "warmup": 0, "ms_per_step": None, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": r["workload"], "events": r["events"]},
+        "batched": r,
+        "roofline": {"bound": "hbm", "achieved": BYTES_PER_CELL_UPDATE * r["cell_updates_per_s"] / 1e9,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": BYTES_PER_CELL_UPDATE * cell_updates / r32["wall"] / 1e9 / HBM_PEAK_GBS,
-                     "traffic": None, "kernel": "batch stage kernels (whole-run average, FP64 storage)"},
+                     "frac": BYTES_PER_CELL_UPDATE * r["cell_updates_per_s"] / 1e9 / HBM_PEAK_GBS,
+                     "traffic": None, "kernel": "batch stage kernels (whole-run average, 856 B per cell update)"},
         "cpu_baseline": None,
     }
     print(json.dumps(line))
 
 
-def serving_requests(V, n, seed):
+def serving_requests(V, n, seed, device_ic=False):
     """Request mix: 80% boost-invariant L x L (L ~ U{121..301}), 20% 3+1D
     L x L x nz (L ~ U{121..201}, nz ~ U{41..81}); dx = deta = 0.1, dtau = 0.02,
-    per-request MC-Glauber seeds.  Arrival times are set per rate."""
+    per-request MC-Glauber seeds.  device_ic: the requests carry only their
+    Glauber geometry and seed and the engine generates each initial state on
+    the device at admission (inside the latency); otherwise the states are
+    built on the host beforehand.  Arrival times are set per rate."""
     serve = importlib.import_module("gpu-vah_amd.serve")
     rng = np.random.default_rng(seed)
     specs = []
@@ -423,8 +361,15 @@ def serving_requests(V, n, seed):
         else:
             L, nz = int(rng.integers(121, 202)), int(rng.integers(41, 82))
         specs.append((i, L, nz, float(rng.uniform(0.0, 12.0)), seed * 100003 + i))
-    es = parallel_map(lambda sp: mc_event(V, sp[1], sp[2], 0.1, 0.1, sp[3], sp[4]), specs)
     base = rng.exponential(1.0, size=n).cumsum()
+    if device_ic:
+        reqs = [serve.Request(i, 0.0, L, L, nz, 0.1, 0.1, 0.02, 0.5, 0.16, None,
+                              glauber=V.glauber(A=208, R=6.62, a=0.546, sigma_nn_mb=70.0, b=b, seed=sd,
+                                                e_peak=60.0 / HBARC, nucleon_width=1.0, eta_flat=1.5, eta_sigma=1.4,
+                                                e_floor=1e-3))
+                for (i, L, nz, b, sd) in specs]
+        return reqs, base
+    es = parallel_map(lambda sp: mc_event(V, sp[1], sp[2], 0.1, 0.1, sp[3], sp[4]), specs)
     reqs = [serve.Request(i, 0.0, L, L, nz, 0.1, 0.1, 0.02, 0.5, 0.16, e) for (i, L, nz, _, _), e in zip(specs, es)]
     return reqs, base
 
@@ -473,14 +418,15 @@ def bench_precision(V, args):
 def bench_serving(V, args):
     serve = importlib.import_module("gpu-vah_amd.serve")
     tab = V.eos_lqcd_table(10000, 1e-8, 1e4)
-    reqs, base = serving_requests(V, args.events, seed=77)
+    reqs, base = serving_requests(V, args.events, seed=77, device_ic=True)
     rates = [float(r) for r in args.rates.split(",")]
+    prec = V.FP32_SPLIT if args.precision == "fp32_split" else V.FP64
     sweeps = []
     for rate in rates:
         for r, a in zip(reqs, base / rate):
             r.arrival, r.admitted, r.finished, r.state = float(a), None, None, -1
         B = V.Batch(budget_bytes=int(args.budget_gb * (1 << 30)), max_events=256, eos_table=tab,
-                    precision=V.FP32_SPLIT, freezeout_T_GeV=T_FREEZE_GEV, steps_per_graph=16)
+                    precision=prec, freezeout_T_GeV=T_FREEZE_GEV, steps_per_graph=16)
         out = serve.serve(B, reqs, steps_per_poll=16)
         B.close()
         out["rate"] = rate
@@ -490,8 +436,7 @@ def bench_serving(V, args):
               f"p99 {out['p99_latency_s']:.2f} s, peak resident {out['peak_resident']}", file=sys.stderr, flush=True)
     # the sequential plain FP64 path: measured service time of every request
     nsteps = sweeps[-1]["steps"]
-    service = [run_plain(V, dict(nx=r.nx, ny=r.ny, nz=r.nz, dx=r.dx, dz=r.dz, dt=r.dt, t0=r.t0, etabar=r.etabar,
-                                 e=r.e), k, tab)[0] for r, k in zip(reqs, nsteps)]
+    service = [plain_service(V, r, k, tab) for r, k in zip(reqs, nsteps)]
     for sw in sweeps:
         lat = serve.fifo_latencies(base / sw["rate"], service)
         finish = max(a + l for a, l in zip(sorted(base / sw["rate"]), lat))
@@ -510,12 +455,12 @@ def bench_serving(V, args):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32/f64",
+        "dtype": "f64" if args.precision == "fp64" else "f32/f64",
         "data": "synthetic",
         "config": {"workload": f"serving: {len(reqs)} Poisson requests, 80% boost-invariant L^2 (L U{{121..301}}), "
                                "20% 3+1D L^2 x nz (L U{121..201}, nz U{41..81}), dx=deta=0.1, dtau=0.02, "
-                               "MC-Glauber (1 fm nucleon width) per-request seeds, retire at max T < 150 MeV, "
-                               "split precision, "
+                               "MC-Glauber (1 fm nucleon width) per-request seeds generated on the device at "
+                               f"admission, retire at max T < 150 MeV, {args.precision}, C++ request loop, "
                                f"HBM budget {args.budget_gb} GiB",
                    "rates_per_s": rates},
         "latency_s": {"p50": top["p50_latency_s"], "p99": top["p99_latency_s"]},

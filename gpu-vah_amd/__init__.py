@@ -85,6 +85,20 @@ class BatchStatus(ctypes.Structure):
                 ("recovery_fallbacks", ctypes.c_longlong), ("nan_cells", ctypes.c_longlong)]
 
 
+class ServeResult(ctypes.Structure):
+    """vah_request_result of include/vah_serve.h."""
+    _fields_ = [("tag", ctypes.c_longlong), ("slot", ctypes.c_int), ("state", ctypes.c_int),
+                ("nstep", ctypes.c_longlong), ("tau", ctypes.c_double), ("admitted_s", ctypes.c_double),
+                ("finished_s", ctypes.c_double), ("latency_s", ctypes.c_double)]
+
+
+class ServeStats(ctypes.Structure):
+    """vah_serve_stats of include/vah_serve.h."""
+    _fields_ = [("served", ctypes.c_int), ("rejected", ctypes.c_int), ("peak_resident", ctypes.c_int),
+                ("polls", ctypes.c_int), ("seconds", ctypes.c_double), ("events_per_s", ctypes.c_double),
+                ("p50_latency_s", ctypes.c_double), ("p99_latency_s", ctypes.c_double)]
+
+
 class BatchVariant(ctypes.Structure):
     _fields_ = [("tile_x", ctypes.c_int), ("tile_y", ctypes.c_int), ("blocks_per_cu", ctypes.c_int),
                 ("kz", ctypes.c_int)]
@@ -104,6 +118,29 @@ class Glauber(ctypes.Structure):
                 ("sigma_nn_mb", ctypes.c_double), ("b", ctypes.c_double), ("alpha", ctypes.c_double),
                 ("e_peak", ctypes.c_double), ("nucleon_width", ctypes.c_double), ("seed", ctypes.c_ulonglong),
                 ("eta_flat", ctypes.c_double), ("eta_sigma", ctypes.c_double), ("e_floor", ctypes.c_double)]
+
+
+class ServeRequest(ctypes.Structure):
+    """vah_request of include/vah_serve.h (with its vah_glauber member)."""
+    _fields_ = [("ev", BatchEvent), ("arrival_s", ctypes.c_double), ("e", ctypes.POINTER(ctypes.c_double)),
+                ("glauber", Glauber), ("pl_ratio", ctypes.c_double)]
+
+
+RETIRE_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.POINTER(ServeResult))
+ADMIT_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(ServeRequest), ctypes.POINTER(ctypes.c_int))
+STEP_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int)
+POLL_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(BatchStatus), ctypes.c_int,
+                           ctypes.POINTER(ctypes.c_int))
+RETIRE_SLOT_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int)
+NOW_FN = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.c_void_p)
+SLEEP_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_double)
+ON_POLL_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_int)
+
+
+class ServeBackend(ctypes.Structure):
+    """vah_serve_backend of include/vah_serve.h."""
+    _fields_ = [("user", ctypes.c_void_p), ("admit", ADMIT_FN), ("step", STEP_FN), ("poll", POLL_FN),
+                ("retire", RETIRE_SLOT_FN), ("now", NOW_FN), ("sleep", SLEEP_FN), ("on_poll", ON_POLL_FN)]
 
 
 class Config(ctypes.Structure):
@@ -144,7 +181,7 @@ def lib():
         L.vah_get_state.argtypes = [ctypes.c_void_p] + [_dp] * 5
         L.vah_get_time.argtypes = [ctypes.c_void_p, _dp, ctypes.POINTER(ctypes.c_longlong)]
         L.vah_reduce.argtypes = [ctypes.c_void_p, ctypes.POINTER(Stats)]
-        L.vah_validity.argtypes = [ctypes.c_void_p] + [_dp] * 4
+        L.vah_validity.argtypes = [ctypes.c_void_p] + [_dp] * 5
         L.vah_profile_enable.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.vah_profile_read.argtypes = [ctypes.c_void_p, ctypes.POINTER(Profile)]
         L.vah_destroy.argtypes = [ctypes.c_void_p]
@@ -170,6 +207,13 @@ def lib():
         L.vah_batch_create.argtypes = [ctypes.POINTER(BatchOptions), ctypes.POINTER(ctypes.c_void_p)]
         L.vah_batch_admit.argtypes = [ctypes.c_void_p, ctypes.POINTER(BatchEvent)] + [_dp] * 7 + \
                                      [ctypes.POINTER(ctypes.c_int)]
+        L.vah_batch_admit_glauber.argtypes = [ctypes.c_void_p, ctypes.POINTER(BatchEvent), ctypes.POINTER(Glauber),
+                                              ctypes.c_double, ctypes.POINTER(ctypes.c_int)]
+        L.vah_serve.argtypes = [ctypes.c_void_p, ctypes.POINTER(ServeRequest), ctypes.c_int, ctypes.c_int, RETIRE_FN,
+                                ctypes.c_void_p, ctypes.POINTER(ServeResult), ctypes.POINTER(ServeStats)]
+        L.vah_serve_run.argtypes = [ctypes.POINTER(ServeBackend), ctypes.POINTER(ServeRequest), ctypes.c_int,
+                                    ctypes.c_int, RETIRE_FN, ctypes.c_void_p, ctypes.POINTER(ServeResult),
+                                    ctypes.POINTER(ServeStats)]
         L.vah_batch_step.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.vah_batch_synchronize.argtypes = [ctypes.c_void_p]
         L.vah_batch_poll.argtypes = [ctypes.c_void_p, ctypes.POINTER(BatchStatus), ctypes.c_int,
@@ -318,9 +362,10 @@ class Vah:
         return {"q": q.reshape(NQ, n), "e": e, "p": p, "u": u.reshape(4, n), "up": up.reshape(4, n)}
 
     def validity(self):
-        """Knudsen numbers (theta_T, zDzu, theta) and R_pi after the last step."""
-        out = {k: np.empty(self.n) for k in ("kn_T", "kn_L", "kn", "r_pi")}
-        _check(lib().vah_validity(self.h, *[out[k].ctypes.data_as(_dp) for k in ("kn_T", "kn_L", "kn", "r_pi")]))
+        """Knudsen numbers (theta_T, zDzu, theta), R_pi and R_pi2 after the last step."""
+        keys = ("kn_T", "kn_L", "kn", "r_pi", "r_pi2")
+        out = {k: np.empty(self.n) for k in keys}
+        _check(lib().vah_validity(self.h, *[out[k].ctypes.data_as(_dp) for k in keys]))
         return out
 
     @property
@@ -410,6 +455,20 @@ class Batch:
         args = [_arr(x) for x in (e, pl, ut, ux, uy, un, pi)]
         slot = ctypes.c_int(-1)
         rc = lib().vah_batch_admit(self.h, ctypes.byref(ev), *[a[1] for a in args], ctypes.byref(slot))
+        if rc in (VAH_ERR_BUDGET, VAH_ERR_FULL) and not raise_on_full:
+            return None
+        _check(rc)
+        self.dims[slot.value] = (nx, ny, nz)
+        return slot.value
+
+    def admit_glauber(self, nx, ny, nz, dx, dy, dz, dt, t0, etabar, g, pl_ratio=1.0 / 3.0, max_steps=0, tag=0,
+                      raise_on_full=True):
+        """Admit one event whose Monte-Carlo Glauber initial state (vah_glauber g)
+        is generated on the device; P_L = pl_ratio e.  Slot, or None when full
+        and raise_on_full is False."""
+        ev = BatchEvent(nx, ny, nz, dx, dy, dz, dt, t0, etabar, max_steps, tag)
+        slot = ctypes.c_int(-1)
+        rc = lib().vah_batch_admit_glauber(self.h, ctypes.byref(ev), ctypes.byref(g), pl_ratio, ctypes.byref(slot))
         if rc in (VAH_ERR_BUDGET, VAH_ERR_FULL) and not raise_on_full:
             return None
         _check(rc)

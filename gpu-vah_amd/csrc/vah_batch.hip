@@ -25,7 +25,9 @@
 #include <vector>
 
 #include "../../include/vah_batch.h"
+#include "../host/ic_internal.h"
 #include "vah_batch_kernels.h"
+#include "vah_glauber.h"
 #include "vah_host_common.h"
 
 using namespace vah;
@@ -519,37 +521,29 @@ int vah_batch_create(const vah_batch_options *opt_in, vah_batch **out) {
     return VAH_OK;
 }
 
-int vah_batch_admit(vah_batch *b, const vah_batch_event *d, const double *e, const double *pl, const double *ut,
-                    const double *ux, const double *uy, const double *un, const double *pi, int *slot) {
-    if (!b || !d || !e || !slot) return fail(VAH_ERR_ARG, "vah_batch_admit: NULL argument");
-    *slot = -1;
-    const int nu = (ut != nullptr) + (ux != nullptr) + (uy != nullptr) + (un != nullptr);
-    if (nu != 0 && nu != 4) return fail(VAH_ERR_ARG, "vah_batch_admit: give all four u^mu arrays or none");
+}  // extern "C"
+
+namespace {
+
+// Checks an event's description, takes a free slot and an arena region for it
+// and fills its descriptor; the slot is not marked used yet.
+int reserve_event(vah_batch *b, const vah_batch_event *d, const char *who, int *slot_out) {
     if (d->nx <= 0 || d->ny <= 0 || d->nz <= 0 || !(d->dx > 0) || !(d->dy > 0) || !(d->dz > 0) || !(d->dt > 0))
-        return fail(VAH_ERR_ARG, "vah_batch_admit: lattice sizes and spacings must be positive");
-    if (!(d->t0 > 0) || !(d->etabar > 0)) return fail(VAH_ERR_ARG, "vah_batch_admit: need t0 > 0 and eta/s > 0");
+        return fail(VAH_ERR_ARG, "%s: lattice sizes and spacings must be positive", who);
+    if (!(d->t0 > 0) || !(d->etabar > 0)) return fail(VAH_ERR_ARG, "%s: need t0 > 0 and eta/s > 0", who);
     int s = -1;
     for (int k = 0; k < b->max_events; ++k)
         if (!b->events[k].used) {
             s = k;
             break;
         }
-    if (s < 0) return fail(VAH_ERR_FULL, "vah_batch_admit: all %d event slots are taken", b->max_events);
+    if (s < 0) return fail(VAH_ERR_FULL, "%s: all %d event slots are taken", who, b->max_events);
     if ((double)vah_host::Layout(d->nx, d->ny, d->nz).fs * NQ * 8 >= 4294967296.0)
-        return fail(VAH_ERR_ARG, "vah_batch_admit: a 15-field set of this lattice exceeds 4 GB");
-    HIP_TRY(hipSetDevice(b->device));
-    const long n = (long)d->nx * d->ny * d->nz;
-    const int nin = 1 + (pl ? 1 : 0) + (nu ? 4 : 0) + (pi ? 10 : 0);
-    const size_t in_bytes = sizeof(double) * (size_t)nin * n;
-    const size_t desc_off = (in_bytes + 255) / 256 * 256;
-    const size_t dyn_off = desc_off + (sizeof(BatchEvent) + 255) / 256 * 256;
-    vah_batch::Staging *st = nullptr;
-    int rc;
-    if ((rc = staging(b, dyn_off + sizeof(BatchDyn), &st)) != VAH_OK) return rc;
+        return fail(VAH_ERR_ARG, "%s: a 15-field set of this lattice exceeds 4 GB", who);
     const size_t bytes = event_bytes(d->nx, d->ny, d->nz);
     size_t off = 0;
     if (!arena_alloc(b, bytes, &off))
-        return fail(VAH_ERR_BUDGET, "vah_batch_admit: %.1f MB do not fit (%.1f of %.1f MB in use)", bytes / 1e6,
+        return fail(VAH_ERR_BUDGET, "%s: %.1f MB do not fit (%.1f of %.1f MB in use)", who, bytes / 1e6,
                     b->used_bytes / 1e6, b->arena_bytes / 1e6);
     HostEvent &h = b->events[s];
     h.desc = *d;
@@ -557,12 +551,80 @@ int vah_batch_admit(vah_batch *b, const vah_batch_event *d, const double *e, con
     h.offset = off;
     h.bytes = bytes;
     fill_descriptor(b, h);
-    auto undo = [&](int code) {
-        arena_free(b, h.offset, h.bytes);
-        h.used = false;
-        return code;
-    };
+    *slot_out = s;
+    return VAH_OK;
+}
 
+void release_event(vah_batch *b, int s) {
+    HostEvent &h = b->events[s];
+    arena_free(b, h.offset, h.bytes);
+    h.used = false;
+}
+
+// Level-0 state of event s from compact inputs already on the device at
+// dev_in (init_event_kernel's layout), its ghosts, and the upload of its
+// descriptor and dynamic record from the staging buffer; marks the slot used.
+int finish_admission(vah_batch *b, int s, const double *dev_in, int has_pl, int has_u, int has_pi,
+                     vah_batch::Staging *st, size_t desc_off, size_t dyn_off, const char *who) {
+    HostEvent &h = b->events[s];
+    const vah_batch_event *d = &h.desc;
+    const long n = (long)d->nx * d->ny * d->nz;
+    BatchDyn dyn;
+    memset(&dyn, 0, sizeof dyn);
+    dyn.live = 1;
+    dyn.state = EV_LIVE;
+    memcpy(st->host + desc_off, &h.dev, sizeof(BatchEvent));
+    memcpy(st->host + dyn_off, &dyn, sizeof(BatchDyn));
+    const int blocks = (int)((n + 255) / 256);
+    hipLaunchKernelGGL(init_event_kernel, dim3(blocks), dim3(256), 0, b->stream, dev_in, has_pl, has_u, has_pi,
+                       d->nx, d->ny, d->nz, (long)h.lay.ncx, (long)h.lay.ncx * h.lay.ncy, h.lay.fs, d->t0, b->eos,
+                       h.dev.q[0], h.dev.e[0], h.dev.p[0], h.dev.u[0], h.dev.u[1]);
+    hipLaunchKernelGGL(ghost_fill_kernel, dim3(blocks), dim3(256), 0, b->stream, h.dev.q[0], h.dev.e[0], h.dev.p[0],
+                       h.dev.u[0], h.lay.fs, d->nx, d->ny, d->nz, (long)h.lay.ncx, (long)h.lay.ncx * h.lay.ncy,
+                       d->nz > 1 ? 1 : 0);
+    if (hipGetLastError() != hipSuccess) {
+        release_event(b, s);
+        return fail(VAH_ERR_HIP, "%s: initial-state launch", who);
+    }
+    if (hipMemcpyAsync(b->ev_dev + s, st->host + desc_off, sizeof(BatchEvent), hipMemcpyHostToDevice, b->stream) !=
+            hipSuccess ||
+        hipMemcpyAsync(b->dyn_dev + s, st->host + dyn_off, sizeof(BatchDyn), hipMemcpyHostToDevice, b->stream) !=
+            hipSuccess ||
+        hipEventRecord(st->done, b->stream) != hipSuccess) {
+        release_event(b, s);
+        return fail(VAH_ERR_HIP, "%s: descriptor upload", who);
+    }
+    st->pending = true;
+    h.used = true;
+    h.stopped = false;
+    b->lists_dirty = true;
+    return VAH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vah_batch_admit(vah_batch *b, const vah_batch_event *d, const double *e, const double *pl, const double *ut,
+                    const double *ux, const double *uy, const double *un, const double *pi, int *slot) {
+    if (!b || !d || !e || !slot) return fail(VAH_ERR_ARG, "vah_batch_admit: NULL argument");
+    *slot = -1;
+    const int nu = (ut != nullptr) + (ux != nullptr) + (uy != nullptr) + (un != nullptr);
+    if (nu != 0 && nu != 4) return fail(VAH_ERR_ARG, "vah_batch_admit: give all four u^mu arrays or none");
+    HIP_TRY(hipSetDevice(b->device));
+    const long n = d->nx > 0 && d->ny > 0 && d->nz > 0 ? (long)d->nx * d->ny * d->nz : 0;
+    const int nin = 1 + (pl ? 1 : 0) + (nu ? 4 : 0) + (pi ? 10 : 0);
+    const size_t in_bytes = sizeof(double) * (size_t)nin * n;
+    const size_t desc_off = (in_bytes + 255) / 256 * 256;
+    const size_t dyn_off = desc_off + (sizeof(BatchEvent) + 255) / 256 * 256;
+    int rc, s = -1;
+    if ((rc = reserve_event(b, d, "vah_batch_admit", &s)) != VAH_OK) return rc;
+    vah_batch::Staging *st = nullptr;
+    if ((rc = staging(b, dyn_off + sizeof(BatchDyn), &st)) != VAH_OK) {
+        release_event(b, s);
+        return rc;
+    }
+    HostEvent &h = b->events[s];
     // compact inputs into pinned memory, then everything else on the device:
     // the inputs land in the event's own q* and q^{n+1} sets (free until its
     // first step), a kernel maps them to the padded level-0 state
@@ -583,33 +645,89 @@ int vah_batch_admit(vah_batch *b, const vah_batch_event *d, const double *e, con
         memcpy(col, pi, sizeof(double) * 10 * n);
         col += 10 * n;
     }
-    BatchDyn dyn;
-    memset(&dyn, 0, sizeof dyn);
-    dyn.live = 1;
-    dyn.state = EV_LIVE;
-    memcpy(st->host + desc_off, &h.dev, sizeof(BatchEvent));
-    memcpy(st->host + dyn_off, &dyn, sizeof(BatchDyn));
     double *dev_in = h.dev.q[1];  // 2 x 15 fields of fs >= nin * n values
-    const int blocks = (int)((n + 255) / 256);
-    if (hipMemcpyAsync(dev_in, st->host, in_bytes, hipMemcpyHostToDevice, b->stream) != hipSuccess)
-        return undo(fail(VAH_ERR_HIP, "vah_batch_admit: input upload"));
-    hipLaunchKernelGGL(init_event_kernel, dim3(blocks), dim3(256), 0, b->stream, dev_in, pl ? 1 : 0, nu ? 1 : 0,
-                       pi ? 1 : 0, d->nx, d->ny, d->nz, (long)h.lay.ncx, (long)h.lay.ncx * h.lay.ncy, h.lay.fs, d->t0,
-                       b->eos, h.dev.q[0], h.dev.e[0], h.dev.p[0], h.dev.u[0], h.dev.u[1]);
-    hipLaunchKernelGGL(ghost_fill_kernel, dim3(blocks), dim3(256), 0, b->stream, h.dev.q[0], h.dev.e[0], h.dev.p[0],
-                       h.dev.u[0], h.lay.fs, d->nx, d->ny, d->nz, (long)h.lay.ncx, (long)h.lay.ncx * h.lay.ncy,
-                       d->nz > 1 ? 1 : 0);
-    if (hipGetLastError() != hipSuccess) return undo(fail(VAH_ERR_HIP, "vah_batch_admit: initial-state launch"));
-    if (hipMemcpyAsync(b->ev_dev + s, st->host + desc_off, sizeof(BatchEvent), hipMemcpyHostToDevice, b->stream) !=
-            hipSuccess ||
-        hipMemcpyAsync(b->dyn_dev + s, st->host + dyn_off, sizeof(BatchDyn), hipMemcpyHostToDevice, b->stream) !=
-            hipSuccess ||
-        hipEventRecord(st->done, b->stream) != hipSuccess)
-        return undo(fail(VAH_ERR_HIP, "vah_batch_admit: descriptor upload"));
-    st->pending = true;
-    h.used = true;
-    h.stopped = false;
-    b->lists_dirty = true;
+    if (hipMemcpyAsync(dev_in, st->host, in_bytes, hipMemcpyHostToDevice, b->stream) != hipSuccess) {
+        release_event(b, s);
+        return fail(VAH_ERR_HIP, "vah_batch_admit: input upload");
+    }
+    if ((rc = finish_admission(b, s, dev_in, pl ? 1 : 0, nu ? 1 : 0, pi ? 1 : 0, st, desc_off, dyn_off,
+                               "vah_batch_admit")) != VAH_OK)
+        return rc;
+    *slot = s;
+    return VAH_OK;
+}
+
+int vah_batch_admit_glauber(vah_batch *b, const vah_batch_event *d, const vah_glauber *g, double pl_ratio,
+                            int *slot) {
+    static const char *who = "vah_batch_admit_glauber";
+    if (!b || !d || !g || !slot) return fail(VAH_ERR_ARG, "%s: NULL argument", who);
+    *slot = -1;
+    if (g->A < 1 || g->A > 1000 || !(g->R > 0) || !(g->a > 0) || !(g->nucleon_width > 0) || !(g->sigma_nn_mb >= 0))
+        return fail(VAH_ERR_ARG, "%s: need 1 <= A <= 1000, R, a, nucleon width > 0, sigma_NN >= 0", who);
+    HIP_TRY(hipSetDevice(b->device));
+    // the host part: 2A nucleon positions from the seed (vah_ic_mc_glauber's draws)
+    std::vector<double> xa, ya, xb, yb;
+    if (vah_host::mc_glauber_nucleons(*g, xa, ya, xb, yb) != VAH_OK) return fail(VAH_ERR_ARG, "%s: geometry", who);
+    const int A = g->A;
+    if (d->nx <= 0 || d->ny <= 0 || d->nz <= 0) return fail(VAH_ERR_ARG, "%s: lattice sizes must be positive", who);
+    const long n = (long)d->nx * d->ny * d->nz, nT = (long)d->nx * d->ny;
+    // staging: positions (4A), eta factors (nz), descriptor, dynamic record
+    const size_t pos_bytes = sizeof(double) * (4 * (size_t)A + d->nz);
+    const size_t desc_off = (pos_bytes + 255) / 256 * 256;
+    const size_t dyn_off = desc_off + (sizeof(BatchEvent) + 255) / 256 * 256;
+    int rc, s = -1;
+    if ((rc = reserve_event(b, d, who, &s)) != VAH_OK) return rc;
+    vah_batch::Staging *st = nullptr;
+    if ((rc = staging(b, dyn_off + sizeof(BatchDyn), &st)) != VAH_OK) {
+        release_event(b, s);
+        return rc;
+    }
+    HostEvent &h = b->events[s];
+    // device scratch inside the event's q* and q^{n+1} sets (2 x 15 fields, free
+    // until the first step): [e | P_L] (2n), e_T (nT), max, positions, eta
+    // factors, wounded flags
+    const size_t align = 32;  // doubles
+    const size_t o_eT = ((size_t)2 * n + align - 1) / align * align;
+    const size_t o_max = (o_eT + nT + align - 1) / align * align;
+    const size_t o_pos = o_max + align;
+    const size_t o_flag = (o_pos + 4 * (size_t)A + d->nz + align - 1) / align * align;
+    const size_t need = o_flag + (2 * (size_t)A + 1) / 2 + 1;
+    if (need > (size_t)2 * NQ * h.lay.fs) {
+        release_event(b, s);
+        return fail(VAH_ERR_ARG, "%s: lattice too small for the generator's scratch (%zu of %ld values)", who, need,
+                    2L * NQ * h.lay.fs);
+    }
+    double *base = h.dev.q[1];
+    double *dev_in = base, *eT = base + o_eT, *pos = base + o_pos, *fl = pos + 4 * A;
+    unsigned long long *emax = (unsigned long long *)(base + o_max);
+    int *wounded = (int *)(base + o_flag);
+    double *hp = (double *)st->host;
+    memcpy(hp, xa.data(), sizeof(double) * A);
+    memcpy(hp + A, ya.data(), sizeof(double) * A);
+    memcpy(hp + 2 * A, xb.data(), sizeof(double) * A);
+    memcpy(hp + 3 * A, yb.data(), sizeof(double) * A);
+    for (int k = 0; k < d->nz; ++k) {
+        const double eta = (k - (d->nz - 1) / 2.) * d->dz;
+        hp[4 * A + k] = d->nz > 1 ? vah_ic_eta_profile(eta, g->eta_flat, g->eta_sigma) : 1.0;
+    }
+    if (hipMemcpyAsync(pos, st->host, pos_bytes, hipMemcpyHostToDevice, b->stream) != hipSuccess) {
+        release_event(b, s);
+        return fail(VAH_ERR_HIP, "%s: position upload", who);
+    }
+    const double d2 = 0.1 * g->sigma_nn_mb / 3.14159265358979323846;
+    const double w = g->nucleon_width;
+    hipLaunchKernelGGL(glauber_wounded_kernel, dim3((2 * A + 255) / 256), dim3(256), 0, b->stream, pos, A, d2, wounded,
+                       emax);
+    hipLaunchKernelGGL(glauber_deposit_kernel, dim3((unsigned)((nT + 255) / 256)), dim3(256), 0, b->stream, pos,
+                       wounded, A, d->nx, d->ny, d->dx, d->dy, 1 / (2 * w * w), 64 * w * w, eT, emax);
+    hipLaunchKernelGGL(glauber_extrude_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, b->stream, eT, emax,
+                       fl, nT, n, g->e_peak, g->e_floor, pl_ratio, dev_in);
+    if (hipGetLastError() != hipSuccess) {
+        release_event(b, s);
+        return fail(VAH_ERR_HIP, "%s: generator launch", who);
+    }
+    if ((rc = finish_admission(b, s, dev_in, pl_ratio > 0 ? 1 : 0, 0, 0, st, desc_off, dyn_off, who)) != VAH_OK)
+        return rc;
     *slot = s;
     return VAH_OK;
 }

@@ -393,11 +393,107 @@ __global__ void reduce_kernel(const double *q, const double *e, const double *u,
         for (int k = 0; k < 4; ++k) out[blockIdx.x * 4 + k] = sm[k][0];
 }
 
-// First-order validity diagnostics (HydrodynamicValidity.cpp:47-247) with
-// the reference's arguments after a step: t_{n+1} = t_n + dt (rungeKutta2
-// advances t before its checkValidity call), the conserved variables q^n of
-// the step's start, the new e, p, u and u^n as the previous velocity.
-// out[f*n + m] for f = kn_T, kn_L, kn, r_pi (m: interior cell, x fastest).
+// Second-order inverse Reynolds number of checkValidityKernel
+// (HydrodynamicValidity.cpp:106-205 for the shear tensor, :272-387 for Rpi2):
+// sigma^{mu nu}, its transverse-traceless part sigma_T, J = delta_pipi
+// theta_T pi + tau_pipi I4 - lambda_pipi zDzu pi, and
+// sqrt|J:J / sigma_T:sigma_T| / 2 / etabar_TC.  tau_pipi = 4 delta_pipi / 3
+// (the reference's integer 1/2 is 0); the bulk term of J multiplies Pi = 0.
+// pv: P_L, pi^{tau tau} .. pi^{eta eta} of the step's start.
+struct RPi2In {
+    double t, ut, ux, uy, un;
+    double dtut, dtux, dtuy, dtun, dxut, dxux, dxuy, dxun, dyut, dyux, dyuy, dyun, dnut, dnux, dnuy, dnun;
+    double e;
+};
+
+__device__ double validity_rpi2(const RPi2In &v, const double (&pv)[11]) {
+    const double t = v.t, ut = v.ut, ux = v.ux, uy = v.uy, un = v.un;
+    const double t2 = t * t, t3 = t * t2, ut2 = ut * ut, un2 = un * un, it = 1.0 / t, it2 = it * it;
+    const double dut = ut * v.dtut + ux * v.dxut + uy * v.dyut + un * v.dnut;
+    const double dux = ut * v.dtux + ux * v.dxux + uy * v.dyux + un * v.dnux;
+    const double duy = ut * v.dtuy + ux * v.dxuy + uy * v.dyuy + un * v.dnuy;
+    const double dun = ut * v.dtun + ux * v.dxun + uy * v.dyun + un * v.dnun;
+    const double theta = ut * it + v.dtut + v.dxux + v.dyuy + v.dnun;
+    const double th3 = theta / 3;
+    // sigma^{mu nu}
+    const double stt = -t * ut * un2 + (v.dtut - ut * dut) + (ut2 - 1) * th3;
+    const double stx = 0.5 * (-t * un2 * ux + (v.dtux - v.dxut) - (ux * dut + ut * dux)) + ut * ux * th3;
+    const double sty = 0.5 * (-t * un2 * uy + (v.dtuy - v.dyut) - (uy * dut + ut * duy)) + ut * uy * th3;
+    const double stn = -un * (2 * ut2 + t2 * un2) * 0.5 * it + 0.5 * (v.dtun - v.dnut * it2) -
+                       0.5 * (un * dut + ut * dun) + ut * un * th3;
+    const double sxx = -(v.dxux + ux * dux) + (1 + ux * ux) * th3;
+    const double sxy = -0.5 * (v.dxuy + v.dyux) - 0.5 * (uy * dux + ux * duy) + ux * uy * th3;
+    const double sxn = -ut * ux * un * it - 0.5 * (v.dxun + v.dnux * it2) - 0.5 * (un * dux + ux * dun) + ux * un * th3;
+    const double syy = -(v.dyuy + uy * duy) + (1 + uy * uy) * th3;
+    const double syn = -ut * uy * un * it - 0.5 * (v.dyun + v.dnuy * it2) - 0.5 * (un * duy + uy * dun) + uy * un * th3;
+    const double snn = -ut * (1 + 2 * t2 * un2) / t3 - v.dnun * it2 - un * dun + (it2 + un2) * th3;
+    // z, A, B and the transverse projections
+    const double iF = vah::rsqrt(1 + ux * ux + uy * uy);
+    const double z0 = t * un * iF, z3 = ut * it * iF;
+    const double z02 = z0 * z0, z0z3 = z0 * z3, z32 = z3 * z3;
+    const double A = z02 * stt - 2 * t2 * z0z3 * stn + t2 * t2 * z32 * snn;
+    const double B0 = z0 * stt - t2 * z3 * stn, B1 = z0 * stx - t2 * z3 * sxn;
+    const double B2 = z0 * sty - t2 * z3 * syn, B3 = z0 * stn - t2 * z3 * snn;
+    const double thetaT = 2 * th3 + A;
+    const double zDzu = theta - thetaT;
+    const double hA = 0.5 * A;
+    const double sttT = stt + 2 * z0 * B0 + z02 * A - (1 - ut2 + z02) * hA;
+    const double stxT = stx + z0 * B1 + ut * ux * hA;
+    const double styT = sty + z0 * B2 + ut * uy * hA;
+    const double stnT = stn + z0 * B3 + z3 * B0 + z0z3 * A + (ut * un - z0z3) * hA;
+    const double sxxT = sxx + (1 + ux * ux) * hA;
+    const double sxyT = sxy + ux * uy * hA;
+    const double sxnT = sxn + z3 * B1 + ux * un * hA;
+    const double syyT = syy + (1 + uy * uy) * hA;
+    const double synT = syn + z3 * B2 + uy * un * hA;
+    const double snnT = snn + 2 * z3 * B3 + z32 * A + (it2 + un2 - z32) * hA;
+    const double e_s = v.e, pl = pv[0];
+    const double pitt = pv[1], pitx = pv[2], pity = pv[3], pitn = pv[4], pixx = pv[5], pixy = pv[6], pixn = pv[7];
+    const double piyy = pv[8], piyn = pv[9], pinn = pv[10];
+    const Fits f = fits(powers_of(pl / e_s));
+    const double etaBar_TC = ((3 - f.Rtilde) * e_s - 2 * pl) * 0.125;
+    const double delta_pipi = (3 + f.Rgamma) * 0.5;
+    const double tau_pipi = 4 * delta_pipi / 3;
+    const double lambda_pipi = f.Rgamma - 1;
+    const double ps2 = 0.5 * (pitt * sttT - 2 * pitx * stxT - 2 * pity * styT + pixx * sxxT + 2 * pixy * sxyT +
+                              piyy * syyT + 2 * t2 * (pixn * sxnT + piyn * synT - pitn * stnT) + pinn * snnT * t2 * t2);
+    const double I4tt = (pitt * sttT - pitx * stxT - pity * styT - t2 * pitn * stnT) - (1 - ut2 + z02) * ps2;
+    const double I4tx = 0.5 * ((pitt * stxT + pitx * sttT) - (pitx * sxxT + pixx * stxT) - (pity * sxyT + pixy * styT) -
+                               t2 * (pitn * sxnT + pixn * stnT)) + ut * ux * ps2;
+    const double I4ty = 0.5 * ((pitt * styT + pity * sttT) - (pitx * sxyT + pixy * stxT) - (pity * syyT + piyy * styT) -
+                               t2 * (pitn * synT + piyn * stnT)) + ut * uy * ps2;
+    const double I4tn = 0.5 * ((pitt * stnT + pitn * sttT) - (pitx * sxnT + pixn * stxT) - (pity * synT + piyn * styT) -
+                               t2 * (pitn * snnT + pinn * stnT)) + (ut * un - z0z3) * ps2;
+    const double I4xx = (pitx * stxT - pixx * sxxT - pixy * sxyT - t2 * pixn * sxnT) + (1 + ux * ux) * ps2;
+    const double I4xy = 0.5 * ((pitx * styT + pity * stxT) - (pixx * sxyT + pixy * sxxT) - (pixy * syyT + piyy * sxyT) -
+                               t2 * (pixn * synT + piyn * sxnT)) + ux * uy * ps2;
+    const double I4xn = 0.5 * ((pitx * stnT + pitn * stxT) - (pixx * sxnT + pixn * sxxT) - (pixy * synT + piyn * sxyT) -
+                               t2 * (pixn * snnT + pinn * sxnT)) + ux * un * ps2;
+    const double I4yy = (pity * styT - pixy * sxyT - piyy * syyT - t2 * piyn * synT) + (1 + uy * uy) * ps2;
+    const double I4yn = 0.5 * ((pity * stnT + pitn * styT) - (pixy * sxnT + pixn * sxyT) - (piyy * synT + piyn * syyT) -
+                               t2 * (piyn * snnT + pinn * synT)) + uy * un * ps2;
+    const double I4nn = (pitn * stnT - pixn * sxnT - piyn * synT - t2 * pinn * snnT) + (it2 + un2 - z32) * ps2;
+    const double c2 = delta_pipi * thetaT - lambda_pipi * zDzu;
+    const double Jtt = c2 * pitt + tau_pipi * I4tt, Jtx = c2 * pitx + tau_pipi * I4tx;
+    const double Jty = c2 * pity + tau_pipi * I4ty, Jtn = c2 * pitn + tau_pipi * I4tn;
+    const double Jxx = c2 * pixx + tau_pipi * I4xx, Jxy = c2 * pixy + tau_pipi * I4xy;
+    const double Jxn = c2 * pixn + tau_pipi * I4xn, Jyy = c2 * piyy + tau_pipi * I4yy;
+    const double Jyn = c2 * piyn + tau_pipi * I4yn, Jnn = c2 * pinn + tau_pipi * I4nn;
+    auto contract = [t2](double tt, double tx, double ty, double tn, double xx, double xy, double xn, double yy,
+                         double yn, double nn) {
+        return tt * tt + xx * xx + yy * yy + 2 * (xy * xy - tx * tx - ty * ty) +
+               2 * t2 * (xn * xn + yn * yn - tn * tn) + nn * nn * t2 * t2;
+    };
+    const double ss = contract(sttT, stxT, styT, stnT, sxxT, sxyT, sxnT, syyT, synT, snnT);
+    const double JJ = contract(Jtt, Jtx, Jty, Jtn, Jxx, Jxy, Jxn, Jyy, Jyn, Jnn);
+    return __builtin_sqrt(fabs(JJ / ss)) * 0.5 / etaBar_TC;
+}
+
+// Validity diagnostics (HydrodynamicValidity.cpp:47-387) with the reference's
+// arguments after a step: t_{n+1} = t_n + dt (rungeKutta2 advances t before
+// its checkValidity call), the conserved variables q^n of the step's start,
+// the new e, p, u and u^n as the previous velocity.
+// out[f*n + m] for f = kn_T, kn_L, kn, r_pi, r_pi2 (m: interior cell, x fastest).
 __global__ void validity_kernel(const double *qn, const double *e, const double *p, const double *u,
                                 const double *up, long fs, int nx, int ny, int nz, long sy, long sz, double t,
                                 double dt, double etabar, double facX, double facY, double facZ, Eos eos,
@@ -409,44 +505,55 @@ __global__ void validity_kernel(const double *qn, const double *e, const double 
     const int jj = (int)((m / nx) % ny);
     const int kk = (int)(m / ((long)nx * ny));
     const long s = (long)plane_of(kk, nz > 1) * sz + (long)(jj + 2) * sy + (ii + 2);
-    const double *ut_ = u, *ux_ = u + fs, *uy_ = u + 2 * fs, *un_ = u + 3 * fs;
-    const double ut = ut_[s], ux = ux_[s], uy = uy_[s], un = un_[s];
-    const double utp = up[s], unp = up[3 * fs + s];
-    const double dxux = (ux_[s + 1] - ux_[s - 1]) * facX;
-    const double dxut = (ut_[s + 1] - ut_[s - 1]) * facX, dxun = (un_[s + 1] - un_[s - 1]) * facX;
-    const double dyuy = (uy_[s + sy] - uy_[s - sy]) * facY;
-    const double dyut = (ut_[s + sy] - ut_[s - sy]) * facY, dyun = (un_[s + sy] - un_[s - sy]) * facY;
-    double dnut = 0.0, dnun = 0.0;
-    if (nz > 1) {
-        dnut = (ut_[s + sz] - ut_[s - sz]) * facZ;
-        dnun = (un_[s + sz] - un_[s - sz]) * facZ;
+    RPi2In v;
+    v.t = t;
+    v.e = e[s];
+    double uc[4], d[4][4];  // d[axis][component]: axis 0 = tau, 1 = x, 2 = y, 3 = eta
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const double *f = u + c * fs;
+        uc[c] = f[s];
+        d[0][c] = (uc[c] - up[c * fs + s]) / dt;
+        d[1][c] = (f[s + 1] - f[s - 1]) * facX;
+        d[2][c] = (f[s + sy] - f[s - sy]) * facY;
+        d[3][c] = nz > 1 ? (f[s + sz] - f[s - sz]) * facZ : 0.0;
     }
-    const double dtut = (ut - utp) / dt, dtun = (un - unp) / dt;
+    v.ut = uc[0];
+    v.ux = uc[1];
+    v.uy = uc[2];
+    v.un = uc[3];
+    v.dtut = d[0][0]; v.dtux = d[0][1]; v.dtuy = d[0][2]; v.dtun = d[0][3];
+    v.dxut = d[1][0]; v.dxux = d[1][1]; v.dxuy = d[1][2]; v.dxun = d[1][3];
+    v.dyut = d[2][0]; v.dyux = d[2][1]; v.dyuy = d[2][2]; v.dyun = d[2][3];
+    v.dnut = d[3][0]; v.dnux = d[3][1]; v.dnuy = d[3][2]; v.dnun = d[3][3];
+    const double ut = v.ut, ux = v.ux, uy = v.uy, un = v.un;
     const double ut2 = ut * ut, un2 = un * un, t2 = t * t, t3 = t * t2;
-    const double Dut = ut * dtut + ux * dxut + uy * dyut + un * dnut + t * un * un;
+    const double Dut = ut * v.dtut + ux * v.dxut + uy * v.dyut + un * v.dnut + t * un * un;
     const double dut = Dut - t * un * un;
-    const double dun = ut * dtun + ux * dxun + uy * dyun + un * dnun;
-    const double theta = ut / t + dtut + dxux + dyuy + dnun;
-    const double stt = -t * ut * un2 + (dtut - ut * dut) + (ut2 - 1) * theta / 3;
-    const double stn = -un * (2 * ut2 + t2 * un2) / (2 * t) + (dtun - dnut / t2) / 2 - (un * dut + ut * dun) / 2 +
+    const double dun = ut * v.dtun + ux * v.dxun + uy * v.dyun + un * v.dnun;
+    const double theta = ut / t + v.dtut + v.dxux + v.dyuy + v.dnun;
+    const double stt = -t * ut * un2 + (v.dtut - ut * dut) + (ut2 - 1) * theta / 3;
+    const double stn = -un * (2 * ut2 + t2 * un2) / (2 * t) + (v.dtun - v.dnut / t2) / 2 - (un * dut + ut * dun) / 2 +
                        ut * un * theta / 3;
-    const double snn = -ut * (1 + 2 * t2 * un2) / t3 - dnun / t2 - un * dun + (1 / t2 + un2) * theta / 3;
+    const double snn = -ut * (1 + 2 * t2 * un2) / t3 - v.dnun / t2 - un * dun + (1 / t2 + un2) * theta / 3;
     const double F = 1 + ux * ux + uy * uy;
     const double z0 = t * un / __builtin_sqrt(F), z3 = ut / t / __builtin_sqrt(F);
     const double A = z0 * z0 * stt - 2 * t2 * z0 * z3 * stn + t2 * t2 * z3 * z3 * snn;
     const double thetaT = 2 * theta / 3 + A;
     const double zDzu = theta / 3 - A;
-    const double T = eos_T(eos, e[s]);
+    const double T = eos_T(eos, v.e);
     out[m] = 5 * etabar * fabs(thetaT) / T;
     out[n + m] = 5 * etabar * fabs(zDzu) / T;
     out[2 * n + m] = 5 * etabar * fabs(theta) / T;
-    const double pitt = qn[PITT * fs + s], pitx = qn[PITX * fs + s], pity = qn[PITY * fs + s];
-    const double pitn = qn[PITN * fs + s], pixx = qn[PIXX * fs + s], pixy = qn[PIXY * fs + s];
-    const double pixn = qn[PIXN * fs + s], piyy = qn[PIYY * fs + s], piyn = qn[PIYN * fs + s];
-    const double pinn = qn[PINN * fs + s];
+    double pv[11];
+#pragma unroll
+    for (int f = 0; f < 11; ++f) pv[f] = qn[(PL + f) * fs + s];
+    const double pitt = pv[1], pitx = pv[2], pity = pv[3], pitn = pv[4], pixx = pv[5], pixy = pv[6];
+    const double pixn = pv[7], piyy = pv[8], piyn = pv[9], pinn = pv[10];
     const double pipi = pitt * pitt - 2 * pitx * pitx - 2 * pity * pity + pixx * pixx + 2 * pixy * pixy + piyy * piyy -
                         2 * pitn * pitn * t2 + 2 * pixn * pixn * t2 + 2 * piyn * piyn * t2 + pinn * pinn * t2 * t2;
-    out[3 * n + m] = __builtin_sqrt(fabs(pipi)) / fabs(pt_hat(e[s], p[s], qn[PL * fs + s]));
+    out[3 * n + m] = __builtin_sqrt(fabs(pipi)) / fabs(pt_hat(v.e, p[s], pv[0]));
+    out[4 * n + m] = validity_rpi2(v, pv);
 }
 
 }  // namespace
@@ -727,13 +834,13 @@ int vah_reduce(vah_ctx *c, vah_stats *out) {
     return VAH_OK;
 }
 
-int vah_validity(vah_ctx *c, double *kn_T, double *kn_L, double *kn, double *r_pi) {
+int vah_validity(vah_ctx *c, double *kn_T, double *kn_L, double *kn, double *r_pi, double *r_pi2) {
     if (!c) return fail(VAH_ERR_ARG, "vah_validity: NULL context");
     if (!c->stepped)
         return fail(VAH_ERR_STATE, "vah_validity: needs a step completed since the last state upload");
     HIP_TRY(hipSetDevice(c->device));
     const long n = (long)c->lat.nx * c->lat.ny * c->lat.nz;
-    // the stage-1 output set is free between steps: 4 interior-sized outputs fit in it
+    // the stage-1 output set is free between steps: 5 interior-sized outputs fit in it
     double *out = c->qset[c->qs];
     // the reference's t after its step: t_n, as the driver computes it, plus dt
     // (FullyDiscreteKurganovTadmorScheme.cpp:766,791)
@@ -743,8 +850,8 @@ int vah_validity(vah_ctx *c, double *kn_T, double *kn_L, double *kn, double *r_p
                        c->lat.ny, c->lat.nz, (long)c->ncx, (long)c->ncx * c->ncy, t, c->lat.dt, c->hydro.etabar,
                        1 / c->lat.dx / 2, 1 / c->lat.dy / 2, 1 / c->lat.dz / 2, c->eos, out);
     HIP_TRY(hipGetLastError());
-    double *dst[4] = {kn_T, kn_L, kn, r_pi};
-    for (int f = 0; f < 4; ++f)
+    double *dst[5] = {kn_T, kn_L, kn, r_pi, r_pi2};
+    for (int f = 0; f < 5; ++f)
         if (dst[f])
             HIP_TRY(hipMemcpyAsync(dst[f], out + (size_t)f * n, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));

@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "../../include/vah_host.h"
+#include "ic_internal.h"
 
 namespace {
 
@@ -153,6 +154,19 @@ double conformal_e_of_T(double T) { return 13.8997 * (double)powf((float)T, 4.0f
 
 }  // namespace
 
+namespace vah_host {
+
+int mc_glauber_nucleons(const vah_glauber &g, std::vector<double> &xa, std::vector<double> &ya,
+                        std::vector<double> &xb, std::vector<double> &yb) {
+    if (g.A < 1 || !(g.R > 0) || !(g.a > 0)) return VAH_ERR_ARG;
+    Rng rng(g.seed);
+    sample_nucleus(rng, g.A, g.R, g.a, -g.b / 2, xa, ya);
+    sample_nucleus(rng, g.A, g.R, g.a, +g.b / 2, xb, yb);
+    return VAH_OK;
+}
+
+}  // namespace vah_host
+
 extern "C" {
 
 double vah_ic_eta_profile(double eta, double eta_flat, double eta_sigma) {
@@ -184,10 +198,8 @@ int vah_ic_mc_glauber(const vah_lattice *lat, const vah_glauber *g, double *e, i
     if (!lat || !g || !e || lat->nx < 1 || lat->ny < 1 || lat->nz < 1 || g->A < 1 || !(g->R > 0) || !(g->a > 0) ||
         !(g->nucleon_width > 0))
         return VAH_ERR_ARG;
-    Rng rng(g->seed);
     std::vector<double> xa, ya, xb, yb;
-    sample_nucleus(rng, g->A, g->R, g->a, -g->b / 2, xa, ya);
-    sample_nucleus(rng, g->A, g->R, g->a, +g->b / 2, xb, yb);
+    vah_host::mc_glauber_nucleons(*g, xa, ya, xb, yb);
     const double d2 = 0.1 * g->sigma_nn_mb / kPi;
     std::vector<char> wa(g->A, 0), wb(g->A, 0);
     for (int i = 0; i < g->A; ++i)

@@ -156,17 +156,17 @@ int vah_get_time(const vah_ctx *ctx, double *tau, long long *nstep);
  * (rhic/src/HydroPlugin.cpp:156-158,198-219). */
 int vah_reduce(vah_ctx *ctx, vah_stats *out);
 
-/* First-order validity diagnostics after a step, per interior cell: the
- * Knudsen numbers 5 (eta/s) |theta_T| / T, 5 (eta/s) |zDzu| / T,
- * 5 (eta/s) |theta| / T and the inverse Reynolds number sqrt|pi:pi| / |P_T hat|
- * (knudsenNumberTaupiT / TaupiL / Taupi and Rpi of
- * old_cpu-vah_code/src/HydrodynamicValidity.cpp:195-247, evaluated with the
- * arguments of its call at FullyDiscreteKurganovTadmorScheme.cpp:791: the
- * time after the step, t_n + dt, the conserved variables of the step's
- * start, the new e, p, u).  Any output may be NULL.  Needs a step completed
- * since the last vah_set_state / vah_set_primitives (VAH_ERR_STATE
- * otherwise). */
-int vah_validity(vah_ctx *ctx, double *kn_T, double *kn_L, double *kn, double *r_pi);
+/* Validity diagnostics after a step, per interior cell: the Knudsen numbers
+ * 5 (eta/s) |theta_T| / T, 5 (eta/s) |zDzu| / T, 5 (eta/s) |theta| / T, the
+ * inverse Reynolds number sqrt|pi:pi| / |P_T hat| and the second-order one
+ * sqrt|J:J / sigma_T:sigma_T| / (2 etabar_TC) (knudsenNumberTaupiT / TaupiL /
+ * Taupi, Rpi and Rpi2 of old_cpu-vah_code/src/HydrodynamicValidity.cpp:195-387,
+ * evaluated with the arguments of its call at
+ * FullyDiscreteKurganovTadmorScheme.cpp:791: the time after the step,
+ * t_n + dt, the conserved variables of the step's start, the new e, p, u).
+ * Any output may be NULL.  Needs a step completed since the last
+ * vah_set_state / vah_set_primitives (VAH_ERR_STATE otherwise). */
+int vah_validity(vah_ctx *ctx, double *kn_T, double *kn_L, double *kn, double *r_pi, double *r_pi2);
 
 /* Per-launch HIP-event timing of the two stage kernels (off by default); the
  * reference times each step with a host stopwatch (rhic/src/HydroPlugin.cpp:222-227,234). */

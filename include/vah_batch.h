@@ -28,6 +28,7 @@
 #define VAH_BATCH_H_
 
 #include "vah.h"
+#include "vah_host.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -101,6 +102,21 @@ int vah_batch_create(const vah_batch_options *opt, vah_batch **out);
  * ordered after steps already enqueued.  *slot receives the event's slot. */
 int vah_batch_admit(vah_batch *b, const vah_batch_event *ev, const double *e, const double *pl, const double *ut,
                     const double *ux, const double *uy, const double *un, const double *pi, int *slot);
+
+/* Admits one event whose initial energy density is a Monte-Carlo Glauber
+ * profile generated for it (rhic/src/MonteCarloGlauberModel.cpp:17-108, the
+ * reference's IC type 4, with g's geometry): the 2A nucleon positions are
+ * drawn on the host from g->seed (vah_ic_mc_glauber's generator and draw
+ * order; microseconds), and the wounded-nucleon test, the Gaussian deposits of
+ * the wounded nucleons (cells x nucleons), the scaling to g->e_peak, the eta
+ * profile and the floor run on the device, ordered after steps already
+ * enqueued.  The energy density equals vah_ic_mc_glauber's for the same
+ * lattice and g up to the rounding of exp.  P_L = pl_ratio e (pl_ratio <= 0:
+ * the reference's 0.0154483 e), u at rest, pi = 0.  No counterpart in the
+ * reference, whose driver builds the initial state on the host once per
+ * process; here a request carries only its seed. */
+int vah_batch_admit_glauber(vah_batch *b, const vah_batch_event *ev, const vah_glauber *g, double pl_ratio,
+                            int *slot);
 
 /* Enqueues nsteps Heun steps of every live event; returns without waiting. */
 int vah_batch_step(vah_batch *b, int nsteps);

@@ -66,7 +66,7 @@ def lib():
         L.orc_nan_hits.argtypes = [ctypes.c_void_p]
         L.orc_nan_hits.restype = ctypes.c_long
         L.orc_get_state.argtypes = [ctypes.c_void_p] + [_dp] * 5
-        L.orc_validity.argtypes = [ctypes.c_void_p] + [_dp] * 4
+        L.orc_validity.argtypes = [ctypes.c_void_p] + [_dp] * 5
         L.orc_validity.restype = ctypes.c_int
         L.orc_set_eos_wb.argtypes = [ctypes.c_void_p]
         L.orc_set_eos_wb.restype = None
@@ -190,10 +190,12 @@ class Oracle:
         return lib().orc_nan_hits(self._handle())
 
     def validity(self):
-        """Knudsen numbers (theta_T, zDzu, theta) and R_pi after the last step."""
+        """Knudsen numbers (theta_T, zDzu, theta), R_pi and the second-order
+        R_pi2 after the last step."""
         n = self.n
-        out = {k: np.empty(n) for k in ("kn_T", "kn_L", "kn", "r_pi")}
-        if lib().orc_validity(self._handle(), *[_ptr(out[k]) for k in ("kn_T", "kn_L", "kn", "r_pi")]) != 0:
+        keys = ("kn_T", "kn_L", "kn", "r_pi", "r_pi2")
+        out = {k: np.empty(n) for k in keys}
+        if lib().orc_validity(self._handle(), *[_ptr(out[k]) for k in keys]) != 0:
             raise RuntimeError("validity needs a completed step")
         return out
 

@@ -1028,13 +1028,15 @@ void orc_set_primitives(orc_ctx *c, const double *e, const double *pl, const dou
 }
 
 /* Exact state load: q (15*n), e, p, u (4*n), up (4*n); interior-only. */
+/* Exact state load.  Q, the conserved variables of the last step's start
+ * that orc_validity reads, is set to the loaded q as well. */
 void orc_set_state(orc_ctx *c, long nstep, const double *q, const double *e, const double *p, const double *u,
                    const double *up) {
     long n = (long)c->nx * c->ny * c->nz;
     c->nstep = nstep;
     for (long m = 0; m < n; ++m) {
         long s = interior_index(c, m);
-        for (int f = 0; f < NQ; ++f) c->q[f][s] = q[f * n + m];
+        for (int f = 0; f < NQ; ++f) c->q[f][s] = c->Q[f][s] = q[f * n + m];
         c->e[s] = e[m];
         c->p[s] = p[m];
         for (int d = 0; d < 4; ++d) {
@@ -1062,6 +1064,113 @@ double orc_time(const orc_ctx *c) { return c->t0 + (double)c->nstep * c->dt; }
  * 0 restores the reference's pairing. */
 void orc_set_pairing(orc_ctx *c, int mode) { c->pairing = mode; }
 
+/* Inputs of the second-order inverse Reynolds number: tau, the new u and its
+ * time and space derivatives (the same differences as the Knudsen numbers)
+ * and the new e. */
+typedef struct {
+    double t, ut, ux, uy, un;
+    double dtut, dtux, dtuy, dtun, dxut, dxux, dxuy, dxun, dyut, dyux, dyuy, dyun, dnut, dnux, dnuy, dnun;
+    double e;
+} rpi2_in;
+
+/* Second-order inverse Reynolds number of checkValidityKernel
+ * (HydrodynamicValidity.cpp:106-205 for the shear tensor, :272-387 for Rpi2):
+ * sigma^{mu nu}, its transverse-traceless part sigma_T (projector Xi built
+ * with z^mu), J^{mu nu} = delta_pipi theta_T pi + tau_pipi I4 - lambda_pipi
+ * zDzu pi with I4 = pi.sigma_T symmetrised minus Xi (pi:sigma_T)/2, and
+ * sqrt|J:J / sigma_T:sigma_T| / 2 / etabar_TC with
+ * etabar_TC = ((3 - Rtilde) e - 2 P_L)/8.  The reference's integer 1/2 = 0
+ * makes tau_pipi = 4 delta_pipi / 3 here too (:300); the bulk term of J
+ * multiplies Pi = 0.  pv = P_L then pi^{tau tau} .. pi^{eta eta} of the
+ * conserved variables the reference passes (those of the step's start). */
+static double validity_rpi2(const rpi2_in *v, const double *pv) {
+    const double t = v->t, ut = v->ut, ux = v->ux, uy = v->uy, un = v->un;
+    const double t2 = t * t, t3 = t * t2, ut2 = ut * ut, un2 = un * un;
+    const double Dut = ut * v->dtut + ux * v->dxut + uy * v->dyut + un * v->dnut + t * un * un;
+    const double dut = Dut - t * un * un;
+    const double dux = ut * v->dtux + ux * v->dxux + uy * v->dyux + un * v->dnux;
+    const double duy = ut * v->dtuy + ux * v->dxuy + uy * v->dyuy + un * v->dnuy;
+    const double dun = ut * v->dtun + ux * v->dxun + uy * v->dyun + un * v->dnun;
+    const double theta = ut / t + v->dtut + v->dxux + v->dyuy + v->dnun;
+    /* sigma^{mu nu} (:124-133) */
+    const double stt = -t * ut * un2 + (v->dtut - ut * dut) + (ut2 - 1) * theta / 3;
+    const double stx = -(t * un2 * ux) / 2 + (v->dtux - v->dxut) / 2 - (ux * dut + ut * dux) / 2 + ut * ux * theta / 3;
+    const double sty = -(t * un2 * uy) / 2 + (v->dtuy - v->dyut) / 2 - (uy * dut + ut * duy) / 2 + ut * uy * theta / 3;
+    const double stn = -un * (2 * ut2 + t2 * un2) / (2 * t) + (v->dtun - v->dnut / t2) / 2 - (un * dut + ut * dun) / 2 +
+                       ut * un * theta / 3;
+    const double sxx = -(v->dxux + ux * dux) + (1 + ux * ux) * theta / 3;
+    const double sxy = -(v->dxuy + v->dyux) / 2 - (uy * dux + ux * duy) / 2 + ux * uy * theta / 3;
+    const double sxn = -ut * ux * un / t - (v->dxun + v->dnux / t2) / 2 - (un * dux + ux * dun) / 2 + ux * un * theta / 3;
+    const double syy = -(v->dyuy + uy * duy) + (1 + uy * uy) * theta / 3;
+    const double syn = -ut * uy * un / t - (v->dyun + v->dnuy / t2) / 2 - (un * duy + uy * dun) / 2 + uy * un * theta / 3;
+    const double snn = -ut * (1 + 2 * t2 * un2) / t3 - v->dnun / t2 - un * dun + (1 / t2 + un2) * theta / 3;
+    /* z^mu, A = z z sigma, B^mu = z_alpha sigma^{alpha mu} (:154-172) */
+    const double F = 1 + ux * ux + uy * uy;
+    const double z0 = t * un / sqrt(F), z3 = ut / t / sqrt(F);
+    const double z02 = z0 * z0, z0z3 = z0 * z3, z32 = z3 * z3;
+    const double A = z02 * stt - 2 * t2 * z0z3 * stn + t2 * t2 * z32 * snn;
+    const double B0 = z0 * stt - t2 * z3 * stn, B1 = z0 * stx - t2 * z3 * sxn;
+    const double B2 = z0 * sty - t2 * z3 * syn, B3 = z0 * stn - t2 * z3 * snn;
+    const double thetaT = 2 * theta / 3 + A;
+    const double zDzu = theta - thetaT;
+    /* sigma_T^{mu nu} (:186-195) */
+    const double sttT = stt + 2 * z0 * B0 + z02 * A - 0.5 * (1 - ut * ut + z02) * A;
+    const double stxT = stx + z0 * B1 - 0.5 * (-ut * ux) * A;
+    const double styT = sty + z0 * B2 - 0.5 * (-ut * uy) * A;
+    const double stnT = stn + z0 * B3 + z3 * B0 + z0z3 * A - 0.5 * (-ut * un + z0z3) * A;
+    const double sxxT = sxx - 0.5 * (-1 - ux * ux) * A;
+    const double sxyT = sxy - 0.5 * (-ux * uy) * A;
+    const double sxnT = sxn + z3 * B1 - 0.5 * (-ux * un) * A;
+    const double syyT = syy - 0.5 * (-1 - uy * uy) * A;
+    const double synT = syn + z3 * B2 - 0.5 * (-uy * un) * A;
+    const double snnT = snn + 2 * z3 * B3 + z32 * A - 0.5 * (-1 / t2 - un * un + z32) * A;
+    /* coefficients in a = P_L / e (:272-305) */
+    const double e_s = v->e, pl = pv[0];
+    const double pitt = pv[1], pitx = pv[2], pity = pv[3], pitn = pv[4], pixx = pv[5], pixy = pv[6], pixn = pv[7];
+    const double piyy = pv[8], piyn = pv[9], pinn = pv[10];
+    double pw[16];
+    powers(pl / e_s, pw);
+    const double Rtilde = ratfit(RTILDE_N, NTERMS(RTILDE_N), RTILDE_D, NTERMS(RTILDE_D), pw);
+    const double Rgamma = ratfit(RGAMMA_N, NTERMS(RGAMMA_N), RGAMMA_D, NTERMS(RGAMMA_D), pw);
+    const double etaBar_TC = ((3 - Rtilde) * e_s - 2 * pl) / 8;
+    const double delta_pipi = (3 + Rgamma) / 2;
+    const double tau_pipi = 4 * (delta_pipi - 1 / 2) / 3; /* the reference's integer 1/2 == 0 */
+    const double lambda_pipi = Rgamma - 1;
+    /* I4 (:318-338) */
+    const double ux2 = ux * ux, uy2 = uy * uy;
+    const double ps = pitt * sttT - 2 * pitx * stxT - 2 * pity * styT + pixx * sxxT + 2 * pixy * sxyT + piyy * syyT -
+                      2 * pitn * stnT * t2 + 2 * pixn * sxnT * t2 + 2 * piyn * synT * t2 + pinn * snnT * t2 * t2;
+    const double ps2 = ps / 2;
+    const double I4tt = (pitt * sttT - pitx * stxT - pity * styT - t2 * pitn * stnT) - (1 - ut2 + z02) * ps2;
+    const double I4tx = (pitt * stxT + pitx * sttT) / 2 - (pitx * sxxT + pixx * stxT) / 2 -
+                        (pity * sxyT + pixy * styT) / 2 - t2 * (pitn * sxnT + pixn * stnT) / 2 + (ut * ux) * ps2;
+    const double I4ty = (pitt * styT + pity * sttT) / 2 - (pitx * sxyT + pixy * stxT) / 2 -
+                        (pity * syyT + piyy * styT) / 2 - t2 * (pitn * synT + piyn * stnT) / 2 + (ut * uy) * ps2;
+    const double I4tn = (pitt * stnT + pitn * sttT) / 2 - (pitx * sxnT + pixn * stxT) / 2 -
+                        (pity * synT + piyn * styT) / 2 - t2 * (pitn * snnT + pinn * stnT) / 2 + (ut * un - z0z3) * ps2;
+    const double I4xx = (pitx * stxT - pixx * sxxT - pixy * sxyT - t2 * pixn * sxnT) + (1 + ux2) * ps2;
+    const double I4xy = (pitx * styT + pity * stxT) / 2 - (pixx * sxyT + pixy * sxxT) / 2 -
+                        (pixy * syyT + piyy * sxyT) / 2 - t2 * (pixn * synT + piyn * sxnT) / 2 + (ux * uy) * ps2;
+    const double I4xn = (pitx * stnT + pitn * stxT) / 2 - (pixx * sxnT + pixn * sxxT) / 2 -
+                        (pixy * synT + piyn * sxyT) / 2 - t2 * (pixn * snnT + pinn * sxnT) / 2 + (ux * un) * ps2;
+    const double I4yy = (pity * styT - pixy * sxyT - piyy * syyT - t2 * piyn * synT) + (1 + uy2) * ps2;
+    const double I4yn = (pity * stnT + pitn * styT) / 2 - (pixy * sxnT + pixn * sxyT) / 2 -
+                        (piyy * synT + piyn * syyT) / 2 - t2 * (piyn * snnT + pinn * synT) / 2 + (uy * un) * ps2;
+    const double I4nn = (pitn * stnT - pixn * sxnT - piyn * synT - t2 * pinn * snnT) + (1 / t2 + un2 - z32) * ps2;
+    /* J = delta I2 + tau I4 - lambda I5, I2 = theta_T pi, I5 = zDzu pi (:312-365) */
+    const double a2 = delta_pipi * thetaT - lambda_pipi * zDzu;
+    const double Jtt = a2 * pitt + tau_pipi * I4tt, Jtx = a2 * pitx + tau_pipi * I4tx;
+    const double Jty = a2 * pity + tau_pipi * I4ty, Jtn = a2 * pitn + tau_pipi * I4tn;
+    const double Jxx = a2 * pixx + tau_pipi * I4xx, Jxy = a2 * pixy + tau_pipi * I4xy;
+    const double Jxn = a2 * pixn + tau_pipi * I4xn, Jyy = a2 * piyy + tau_pipi * I4yy;
+    const double Jyn = a2 * piyn + tau_pipi * I4yn, Jnn = a2 * pinn + tau_pipi * I4nn;
+    const double ss = sttT * sttT - 2 * stxT * stxT - 2 * styT * styT + sxxT * sxxT + 2 * sxyT * sxyT + syyT * syyT -
+                      2 * stnT * stnT * t2 + 2 * sxnT * sxnT * t2 + 2 * synT * synT * t2 + snnT * snnT * t2 * t2;
+    const double JJ = Jtt * Jtt - 2 * Jtx * Jtx - 2 * Jty * Jty + Jxx * Jxx + 2 * Jxy * Jxy + Jyy * Jyy -
+                      2 * Jtn * Jtn * t2 + 2 * Jxn * Jxn * t2 + 2 * Jyn * Jyn * t2 + Jnn * Jnn * t2 * t2;
+    return sqrt(fabs(JJ / ss)) / 2 / etaBar_TC;
+}
+
 /* First-order validity diagnostics of checkValidityKernel
  * (HydrodynamicValidity.cpp:47-247) with the arguments the reference passes
  * after a step (FullyDiscreteKurganovTadmorScheme.cpp:791): the time
@@ -1071,9 +1180,11 @@ void orc_set_pairing(orc_ctx *c, int mode) { c->pairing = mode; }
  * velocity.  Per interior cell:
  *   kn_T = 5 etabar |theta_T| / T,  kn_L = 5 etabar |zDzu| / T,
  *   kn = 5 etabar |theta| / T,      r_pi = sqrt|pi:pi| / |P_T hat|
- * (knudsenNumberTaupiT / TaupiL / Taupi and Rpi of the reference).  Returns
- * -1 before the first step.  Any output may be NULL. */
-int orc_validity(const orc_ctx *c, double *kn_T, double *kn_L, double *kn, double *r_pi) {
+ * (knudsenNumberTaupiT / TaupiL / Taupi and Rpi of the reference), and the
+ * second-order inverse Reynolds number r_pi2 = sqrt|J:J / sigma_T:sigma_T| /
+ * (2 etabar_TC) of :272-387 (validity_rpi2).  Returns -1 before the first
+ * step.  Any output may be NULL. */
+int orc_validity(const orc_ctx *c, double *kn_T, double *kn_L, double *kn, double *r_pi, double *r_pi2) {
     if (c->nstep < 1) return -1;
     const double t = (c->t0 + (double)(c->nstep - 1) * c->dt) + c->dt;
     const double dt = c->dt, etabar = c->etabar;
@@ -1116,6 +1227,13 @@ int orc_validity(const orc_ctx *c, double *kn_T, double *kn_L, double *kn, doubl
         if (kn_T) kn_T[m] = 5 * etabar * fabs(thetaT) / T;
         if (kn_L) kn_L[m] = 5 * etabar * fabs(zDzu) / T;
         if (kn) kn[m] = 5 * etabar * fabs(theta) / T;
+        if (r_pi2) {
+            const rpi2_in in = {t, ut, ux, uy, un, dtut, dtux, dtuy, dtun, dxut, dxux, dxuy, dxun, dyut, dyux, dyuy,
+                                dyun, dnut, dnux, dnuy, dnun, e_s};
+            double pv[11];
+            for (int f = 0; f < 11; ++f) pv[f] = c->Q[PL + f][s];
+            r_pi2[m] = validity_rpi2(&in, pv);
+        }
         if (r_pi) {
             const double pl = c->Q[PL][s];
             const double pitt = c->Q[PITT][s], pitx = c->Q[PITX][s], pity = c->Q[PITY][s], pitn = c->Q[PITN][s];

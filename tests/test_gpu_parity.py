@@ -107,9 +107,10 @@ def test_lqcd_table_eos_3d(oracle_mod):
 
 
 def test_validity_diagnostics_3d(oracle_mod):
-    """Knudsen and inverse Reynolds maps after 15 steps of a flowing 3+1D state:
-    the GPU's validity diagnostics against the oracle's, per cell within 1e-10
-    of each map's largest value."""
+    """Knudsen and inverse Reynolds maps (first and second order) after 15
+    steps of a flowing 3+1D state: the GPU's validity diagnostics against the
+    oracle's, per cell within 1e-10 of each map's largest value, and R_pi2
+    per cell within 1e-9 of its own value."""
     V = vah()
     nx, ny, nz = 22, 20, 9
     e = gaussian_ic(nx, ny, nz, dz=0.3, seed=12, peak=80.0)
@@ -122,11 +123,14 @@ def test_validity_diagnostics_3d(oracle_mod):
     g.synchronize()
     o.step(15)
     a, b = g.validity(), o.validity()
-    for k in ("kn_T", "kn_L", "kn", "r_pi"):
+    for k in ("kn_T", "kn_L", "kn", "r_pi", "r_pi2"):
         scale = np.max(np.abs(b[k]))
         err = np.max(np.abs(a[k] - b[k])) / scale
         print(k, "max", scale, "worst", err)
         assert scale > 0 and err < 1e-10, k
+    rel = np.abs(a["r_pi2"] / b["r_pi2"] - 1)
+    print("r_pi2 per-cell relative: max", rel.max(), "median", np.median(rel))
+    assert rel.max() < 1e-9
     g.close()
     o.close()
 

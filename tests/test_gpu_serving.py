@@ -1,8 +1,17 @@
-"""GPU test of the serving loop (gpu-vah_amd/serve.py) over the real batch
-engine: a burst of boost-invariant requests into an HBM budget that holds
-only two of the largest at a time.  Every request is admitted first come,
-first served, retired at its freeze-out step (hottest cell below 150 MeV),
-and holds the fields the single-lattice path computes for that many steps."""
+"""GPU tests of the serving form: the C++ request loop (include/vah_serve.h,
+through gpu-vah_amd/serve.py) over the real batch engine, and the
+Monte-Carlo Glauber initial states the engine generates on the device at
+admission (vah_batch_admit_glauber).
+
+- a burst of boost-invariant requests into an HBM budget that holds only two
+  of the largest at a time: every request is admitted first come, first
+  served, retired at its freeze-out step (hottest cell below 150 MeV), and
+  holds the fields the single-lattice path computes for that many steps;
+- device-generated initial states equal the host generator's
+  (vah_ic_mc_glauber) for the same seed, in 2+1D and 3+1D;
+- requests that carry only a seed (one of them 3+1D) are served, each
+  result equal to the single-lattice path started from the host-generated
+  state."""
 import importlib
 
 import numpy as np
@@ -38,7 +47,7 @@ def test_serving_loop_retires_every_request_at_its_freeze_out():
     for r in reqs:
         assert r.state == V.EVENT_FROZEN and 1 < r.nstep < r.max_steps
         g = V.Vah(r.nx, r.ny, 1, r.dx, r.dx, r.dz, r.dt, r.etabar, r.t0, eos_table=tab)
-        g.set_primitives(r.e, *rest_frame(r.cells), pl=r.e / 3)
+        g.set_primitives(r.e, *rest_frame(r.cells), pl=r.pl)
         g.step(r.nstep - 1)
         assert g.stats().max_T * HBARC >= T_F  # still hot one step earlier
         g.step(1)
@@ -48,3 +57,55 @@ def test_serving_loop_retires_every_request_at_its_freeze_out():
         print(f"request {r.rid}: {r.nx}x{r.ny}, frozen after {r.nstep} steps, latency {r.latency * 1e3:.1f} ms, "
               f"worst vs single-lattice path {err:.1e}")
         assert err <= 1e-12
+
+
+def _glauber(V, seed, b, width=1.0):
+    return V.glauber(A=208, R=6.62, a=0.546, sigma_nn_mb=70.0, b=b, seed=seed, e_peak=60.0 / HBARC,
+                     nucleon_width=width, eta_flat=1.5, eta_sigma=1.4, e_floor=1e-3)
+
+
+@pytest.mark.parametrize("dims", [(121, 121, 1, 0.1, 0.1), (61, 57, 15, 0.2, 0.25)], ids=["2+1D", "3+1D"])
+def test_device_generated_glauber_state_equals_the_host_generator(dims):
+    V = vah()
+    nx, ny, nz, dx, dz = dims
+    g = _glauber(V, seed=31, b=5.0)
+    e_host, nw = V.ic_mc_glauber(nx, ny, nz, dx, dz, g)
+    assert nw > 50
+    B = V.Batch(budget_bytes=V.batch_event_bytes(nx, ny, nz) + (8 << 20), max_events=2, freezeout_T_GeV=0.0,
+                steps_per_graph=0)
+    slot = B.admit_glauber(nx, ny, nz, dx, dx, dz, 0.02, 0.5, 0.16, g, pl_ratio=1.0 / 3.0)
+    st = B.state(slot)
+    B.close()
+    rel = np.abs(st["e"] - e_host) / e_host
+    print(f"{nx}x{ny}x{nz}: {nw} wounded, max relative difference {rel.max():.1e}")
+    assert rel.max() < 1e-13
+    assert np.array_equal(st["q"][4], st["e"] * (1.0 / 3.0))  # P_L = e/3 of the device's e
+    assert np.all(st["u"][0] == 1.0) and np.all(st["u"][1:] == 0.0)
+
+
+def test_seed_only_requests_are_served_like_the_single_lattice_path():
+    V = vah()
+    S = importlib.import_module("gpu-vah_amd.serve")
+    tab = V.eos_lqcd_table(10000, 1e-8, 1e4)
+    shapes = [(61, 61, 1, 0.2, 0.1), (41, 41, 9, 0.3, 0.4), (71, 67, 1, 0.2, 0.1)]
+    reqs = []
+    for i, (nx, ny, nz, dx, dz) in enumerate(shapes):
+        reqs.append(S.Request(rid=i, arrival=0.0005 * i, nx=nx, ny=ny, nz=nz, dx=dx, dz=dz, dt=0.2 * dx, t0=0.5,
+                              etabar=0.16, e=None, glauber=_glauber(V, seed=100 + i, b=3.0 + i), max_steps=3000))
+    budget = sum(V.batch_event_bytes(r.nx, r.ny, r.nz) for r in reqs)
+    B = V.Batch(budget_bytes=budget, max_events=4, eos_table=tab, freezeout_T_GeV=T_F, steps_per_graph=8)
+    out = S.serve(B, reqs, steps_per_poll=8, keep_fields=True)
+    B.close()
+    assert out["events"] == len(reqs) and out["rejected"] == 0
+    for r in reqs:
+        assert r.state == V.EVENT_FROZEN and 1 < r.nstep < r.max_steps
+        e, _ = V.ic_mc_glauber(r.nx, r.ny, r.nz, r.dx, r.dz, r.glauber)
+        g = V.Vah(r.nx, r.ny, r.nz, r.dx, r.dx, r.dz, r.dt, r.etabar, r.t0, eos_table=tab)
+        g.set_primitives(e, *rest_frame(r.cells), pl=e * (1.0 / 3.0))
+        g.step(r.nstep)
+        g.synchronize()
+        err = max(field_errors(r.fields, g.state(), r.tau).values())
+        g.close()
+        print(f"request {r.rid}: {r.nx}x{r.ny}x{r.nz} from seed {r.glauber.seed}, frozen after {r.nstep} steps, "
+              f"latency {r.latency * 1e3:.1f} ms, worst vs single lattice {err:.1e}")
+        assert err <= 1e-10

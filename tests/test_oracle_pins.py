@@ -275,3 +275,56 @@ def test_inverse_reynolds_number_by_tensor_contraction(oracle_mod):
     pipi = np.einsum("nab,ac,bd,ncd->n", pi, g, g, pi)
     r = np.sqrt(np.abs(pipi)) / np.abs((st["e"] - q0[4]) / 2)
     np.testing.assert_allclose(v["r_pi"], r, rtol=1e-12)
+
+
+def test_validity_maps_by_tensor_calculus(oracle_mod):
+    """Every validity map on a flowing 3+1D state with u^eta != 0 and pi != 0,
+    against the covariant definitions evaluated in tests/milne.py (index
+    contractions with the Milne metric, not the reference's component
+    expansion): the Knudsen numbers
+    5 (eta/s) |theta_T|, |zDzu|, |theta| / T, and the second-order inverse
+    Reynolds number sqrt|J:J / sigma_T:sigma_T| / (2 etabar_TC) with
+    J = delta_pipi theta_T pi + tau_pipi I4 - lambda_pipi zDzu pi,
+    I4^{mu nu} = (pi^{mu a} sigma_T,a^nu + pi^{nu a} sigma_T,a^mu)/2 -
+    Xi^{mu nu} (pi:sigma_T)/2 (HydrodynamicValidity.cpp:272-387; the
+    reference's integer 1/2 makes tau_pipi = 4 delta_pipi / 3), pi and P_L
+    from the step's start, T, e and u after it."""
+    import ctypes
+    import ctypes.util
+    from tests.milne import contract, kinematics, pi_tensor
+    from tests.parity import gaussian_ic, rest_frame
+    libm = ctypes.CDLL(ctypes.util.find_library("m"))
+    libm.powf.argtypes = [ctypes.c_float, ctypes.c_float]
+    libm.powf.restype = ctypes.c_float
+    nx, ny, nz, dx, dz, dt = 12, 10, 6, 0.1, 0.4, 0.01
+    e0 = gaussian_ic(nx, ny, nz, dz=dz, sx=0.8, sy=1.1, tilt=0.3, peak=60.0, seed=3)
+    n = nx * ny * nz
+    o = oracle_mod.Oracle(nx, ny, nz, dx, dx, dz, dt, ETABAR, 0.5, threads=2)
+    o.set_primitives(e0, *rest_frame(n), pl=e0 / 3)
+    o.step(15)
+    q0 = o.state()["q"]
+    o.step(1)
+    v, st, tau = o.validity(), o.state(), o.time
+    o.close()
+    assert np.max(np.abs(st["u"][3])) > 1e-3 and np.max(np.abs(q0[5:])) > 0
+    theta, theta_T, zDzu, sT, Xi = kinematics(st["u"], st["up"], (nz, ny, nx), tau, dt, dx, dx, dz)
+    T = np.array([libm.powf(ctypes.c_float(x / 13.8997), ctypes.c_float(0.25)) for x in st["e"]], dtype=float)
+    kn = 5 * ETABAR / T
+    for key, val in (("kn_T", theta_T), ("kn_L", zDzu), ("kn", theta)):
+        np.testing.assert_allclose(v[key], kn * np.abs(val), rtol=1e-12, atol=1e-14 * np.max(kn * np.abs(theta)))
+    # second order
+    L = oracle_mod.lib()
+    e, pl = st["e"], q0[4]
+    Rt = np.array([L.orc_rtilde(a) for a in pl / e])
+    Rg = np.array([L.orc_rgamma(a) for a in pl / e])
+    pi = pi_tensor(q0)
+    g = np.diag([1.0, -1.0, -1.0, -tau * tau])
+    ps = contract(pi, sT, tau)
+    I4 = np.einsum("nma,ab,nbv->nmv", pi, g, sT)
+    I4 = 0.5 * (I4 + I4.transpose(0, 2, 1)) - 0.5 * Xi * ps[:, None, None]
+    delta = (3 + Rg) / 2
+    J = (delta * theta_T - (Rg - 1) * zDzu)[:, None, None] * pi + (4 * delta / 3)[:, None, None] * I4
+    etaTC = ((3 - Rt) * e - 2 * pl) / 8
+    r2 = np.sqrt(np.abs(contract(J, J, tau) / contract(sT, sT, tau))) / 2 / etaTC
+    assert np.all(np.isfinite(r2)) and np.max(r2) > 1e-3
+    np.testing.assert_allclose(v["r_pi2"], r2, rtol=1e-12)

@@ -1,6 +1,8 @@
-"""Scheduling logic of the serving loop (gpu-vah_amd/serve.py) on the CPU,
-against a stand-in batch with the engine's admission rules: a byte budget,
-a slot limit, per-event step counts to freeze-out, and a simulated clock."""
+"""Scheduling logic of the serving loop on the CPU: the C++ loop
+(vah_serve_run, include/vah_serve.h, called through gpu-vah_amd/serve.py)
+drives a stand-in batch with the engine's admission rules -- a byte budget,
+a slot limit, per-event step counts to freeze-out -- and a simulated clock,
+both supplied as callbacks."""
 import importlib
 
 import numpy as np

@@ -64,6 +64,11 @@ def initial_state(V, w):
     return e
 
 
+def progress(msg):
+    """One line on stderr per phase (the JSON line stays alone on stdout)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def host_cpu():
     """CPU model and the number of cores this process may run on."""
     model = "unknown"
@@ -248,8 +253,10 @@ def batch_sample(V, n_events, tab, autotune_steps=0, max_steps=1000):
     per-cell error against the FP64 batch at the FP64 freeze-out steps."""
     evs = batch_events(V, n_events, seed=2024)
     cells = np.array([ev["nx"] * ev["ny"] for ev in evs])
+    progress("batched FP64 to freeze-out")
     r64 = run_batched(V, evs, tab, V.FP64, T_FREEZE_GEV, max_steps=[max_steps] * n_events,
                       autotune_steps=autotune_steps)
+    progress("batched split precision")
     nsteps = r64["stop_step"]
     frozen = int(np.sum(np.array(r64["stop_state"]) == V.EVENT_FROZEN))
     r32t = run_batched(V, evs, tab, V.FP32_SPLIT, T_FREEZE_GEV, max_steps=[max_steps] * n_events,
@@ -260,6 +267,7 @@ def batch_sample(V, n_events, tab, autotune_steps=0, max_steps=1000):
                      for a, b in zip(r32["slots"], r64["slots"])])
     r32["B"].close()
     r64["B"].close()
+    progress("plain per-event FP64 path")
     t_plain = sum(run_plain(V, ev, k, tab)[0] for ev, k in zip(evs, nsteps))
     lat = np.array(r64["stop_time"])
     cu = float(np.sum(cells * np.array(nsteps)))
@@ -308,6 +316,7 @@ def serving_sample(V, n_req, rate, tab, budget_gb):
                 freezeout_T_GeV=T_FREEZE_GEV, steps_per_graph=16)
     out = serve.serve(B, reqs, steps_per_poll=16)
     B.close()
+    progress(f"served {out['events']} requests in {out['seconds']:.1f} s; sequential plain FP64 path")
     service = [plain_service(V, r, r.nstep, tab) for r in reqs]
     lat = serve.fifo_latencies(arrivals, service)
     finish = max(a + l for a, l in zip(sorted(arrivals), lat))
@@ -593,6 +602,7 @@ def main():
             k -= m
         return t
 
+    progress(f"{w['name']}: {n} cells, warmup {args.warmup} steps")
     run(args.warmup)
     # timed region: reps x K steps, the median repetition is the value
     g.profile(True)
@@ -600,6 +610,7 @@ def main():
     prof = g.profile_read()
     g.profile(False)
     ms = float(np.median(rep_ms))
+    progress(f"timed {args.reps} x {args.steps} steps: median {ms:.4f} ms/step")
     # one evolution from the initial state, timed in 10-step blocks: the cost
     # of the step as the flow develops (tau0 .. tau0 + reset_steps dtau)
     evolution = None
@@ -656,12 +667,15 @@ def main():
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:
+        progress("CPU baseline")
         line["cpu_baseline"] = cpu_baseline(w, e, eos_table, args.cpu_seconds)
     if args.workload == "mc3d":
         tab = eos_table if eos_table is not None else V.eos_lqcd_table(10000, 1e-8, 1e4)
         if args.batch_events > 0:
+            progress(f"batched sample: {args.batch_events} events")
             line["batched"] = batch_sample(V, args.batch_events, tab)
         if args.serve_requests > 0:
+            progress(f"serving sample: {args.serve_requests} requests at {args.serve_rate:g}/s")
             line["serving"] = serving_sample(V, args.serve_requests, args.serve_rate, tab, args.budget_gb)
     print(json.dumps(line))
 

@@ -27,12 +27,6 @@
 
 namespace vah {
 
-// order of tiles: 0 = (x, y) tiles of one eta chunk consecutive, 1 = eta chunks
-// of one (x, y) tile consecutive
-#ifndef VAH_TILE_ORDER
-#define VAH_TILE_ORDER 0
-#endif
-
 // workgroups per CU the register allocation is sized for (occupancy target)
 #ifndef VAH_BLOCKS_2D
 #define VAH_BLOCKS_2D 2
@@ -458,15 +452,8 @@ __device__ __forceinline__ void stage_tile(const StageParams &P, int tile, R (*H
     };
     static_assert(tile_fits<TX, TY>(), "tile does not fit one workgroup");
     const int tpl = P.tiles_x * P.tiles_y;
-#if VAH_TILE_ORDER == 1
-    // eta chunks of one (x, y) tile are consecutive tiles
-    const int txy = tile / P.zchunks;
-    const int zc = tile - txy * P.zchunks;
-    (void)tpl;
-#else
     const int zc = tile / tpl;
     const int txy = tile - zc * tpl;
-#endif
     const int ty = txy / P.tiles_x;
     const int tx = txy - ty * P.tiles_x;
     const int i0 = tx * TX, j0 = ty * TY;

@@ -104,6 +104,28 @@ class BatchVariant(ctypes.Structure):
                 ("kz", ctypes.c_int)]
 
 
+class DeviceState(ctypes.Structure):
+    """vah_device_state: caller-owned device buffers in the padded layout."""
+    _fields_ = [(k, ctypes.c_void_p) for k in ("q", "q_next", "e", "p", "u", "u_prev")]
+
+
+NOALLOC_SCRATCH_BYTES = 64 << 20
+
+
+def field_stride(nx, ny, nz):
+    """VAH_FIELD_STRIDE: doubles between two fields of the padded device layout."""
+    return ((nx + 4) * (ny + 4) * (nz + 4 if nz > 1 else 1) + 63) // 64 * 64
+
+
+def padded_index(nx, ny, nz):
+    """Padded-layout offset of every interior cell (interior order, x fastest)."""
+    i = np.arange(nx)
+    j = np.arange(ny)
+    k = np.arange(nz) + 2 if nz > 1 else np.zeros(1, dtype=np.int64)
+    K, J, I = np.meshgrid(k, j, i, indexing="ij")
+    return ((K * (ny + 4) + J + 2) * (nx + 4) + I + 2).ravel()
+
+
 class VahError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"vah error {code}: {msg}")
@@ -199,6 +221,9 @@ def lib():
         L.vah_config_load.argtypes = [ctypes.c_char_p, ctypes.POINTER(Config)]
         L.vah_ic_reference.argtypes = [ctypes.POINTER(Config)] + [_dp] * 5
         L.vah_run.argtypes = [ctypes.POINTER(Config), ctypes.c_char_p, ctypes.c_int]
+        L.vah_step_noalloc.argtypes = [ctypes.POINTER(Lattice), ctypes.POINTER(Hydro), ctypes.POINTER(Options),
+                                       ctypes.c_longlong, ctypes.c_int, ctypes.POINTER(DeviceState),
+                                       ctypes.c_void_p]
         # batch engine (include/vah_batch.h)
         L.vah_batch_default_options.argtypes = [ctypes.POINTER(BatchOptions)]
         L.vah_batch_default_options.restype = None

@@ -556,6 +556,61 @@ __global__ void validity_kernel(const double *qn, const double *e, const double 
     out[4 * n + m] = validity_rpi2(v, pv);
 }
 
+// ---- Scratch of the no-allocation step ------------------------------------
+// One file-scope device array per GPU holds the intermediate stage of
+// vah_step_noalloc (q*, e*, p*, u* of the padded lattice) and its NaN /
+// fallback counters; a second one the conformal EOS's powf bitmap.  Both
+// exist on every device the library is loaded on until the process ends.
+// Host code never uses their names as addresses: the device addresses come
+// from hipGetSymbolAddress, once per device, with that device current.
+constexpr size_t kScratchDoubles = VAH_NOALLOC_SCRATCH_BYTES / sizeof(double);
+constexpr int kStageFields = NQ + 2 + 4;  // q*, e*, p*, u*
+static __device__ __align__(256) double scratch[kScratchDoubles];
+static __device__ __align__(256) uint32_t powf_fix_static[1u << 20];  // 2^25 bits
+
+constexpr int kMaxDevices = 64;
+struct NoallocDevice {
+    std::once_flag once;
+    int rc = VAH_OK;
+    std::string err;
+    double *scratch = nullptr;      // device address of `scratch` on this device
+    uint32_t *powf_fix = nullptr;   // device address of `powf_fix_static`
+};
+NoallocDevice g_noalloc[kMaxDevices];
+
+// First call on a device (that device current): the two symbol addresses and
+// the powf bitmap upload.  Synchronous; later calls only enqueue work.
+int noalloc_setup(int dev, NoallocDevice **out) {
+    NoallocDevice &d = g_noalloc[dev];
+    std::call_once(d.once, [&d] {
+        hipError_t e = hipGetSymbolAddress((void **)&d.scratch, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) {
+            d.rc = VAH_ERR_HIP;
+            d.err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+            return;
+        }
+        e = hipGetSymbolAddress((void **)&d.powf_fix, HIP_SYMBOL(powf_fix_static));
+        if (e != hipSuccess) {
+            d.rc = VAH_ERR_HIP;
+            d.err = std::string("hipGetSymbolAddress(powf_fix_static): ") + hipGetErrorString(e);
+            return;
+        }
+        const std::vector<uint32_t> &fix = vah_host::powf_fix_table();
+        e = hipMemcpy(d.powf_fix, fix.data(), fix.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            d.rc = VAH_ERR_HIP;
+            d.err = std::string("powf bitmap upload: ") + hipGetErrorString(e);
+        }
+    });
+    if (d.rc != VAH_OK) return fail(d.rc, "vah_step_noalloc: set-up on device %d: %s", dev, d.err.c_str());
+    *out = &d;
+    return VAH_OK;
+}
+
+__global__ void clear_counters_kernel(unsigned int *c) {
+    if (threadIdx.x < 4) c[threadIdx.x] = 0u;
+}
+
 }  // namespace
 
 extern "C" {
@@ -855,6 +910,109 @@ int vah_validity(vah_ctx *c, double *kn_T, double *kn_L, double *kn, double *r_p
         if (dst[f])
             HIP_TRY(hipMemcpyAsync(dst[f], out + (size_t)f * n, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
+    return VAH_OK;
+}
+
+int vah_step_noalloc(const vah_lattice *lat, const vah_hydro *hydro, const vah_options *opt, long long nstep,
+                     int nsteps, vah_device_state *st, unsigned int *counters) {
+    if (!lattice_ok(lat)) return fail(VAH_ERR_ARG, "vah_step_noalloc: lattice sizes and spacings must be positive");
+    if (!hydro || !(hydro->t0 > 0) || !(hydro->etabar > 0))
+        return fail(VAH_ERR_ARG, "vah_step_noalloc: need t0 > 0 and eta/s > 0");
+    if (!opt || !st) return fail(VAH_ERR_ARG, "vah_step_noalloc: NULL options or state");
+    if (nstep < 0 || nsteps < 0) return fail(VAH_ERR_ARG, "vah_step_noalloc: negative step count");
+    if (!st->q || !st->q_next || !st->e || !st->p || !st->u || !st->u_prev)
+        return fail(VAH_ERR_ARG, "vah_step_noalloc: NULL state buffer");
+    if (opt->precision != VAH_FP64 && opt->precision != VAH_FP32_SPLIT)
+        return fail(VAH_ERR_ARG, "vah_step_noalloc: unknown precision %d", opt->precision);
+    if (opt->flux_mode != VAH_FLUX_SHARED && opt->flux_mode != VAH_FLUX_PER_CELL)
+        return fail(VAH_ERR_ARG, "vah_step_noalloc: unknown flux mode %d", opt->flux_mode);
+    if (opt->precision == VAH_FP32_SPLIT && opt->flux_mode != VAH_FLUX_SHARED)
+        return fail(VAH_ERR_ARG, "vah_step_noalloc: the split-precision form uses shared faces only");
+    if (opt->eta_chunk < 0 || opt->eta_chunk > 256)
+        return fail(VAH_ERR_ARG, "vah_step_noalloc: eta_chunk must be in [0, 256]");
+    if (opt->eos == VAH_EOS_TABLE &&
+        (opt->table.n < 2 || !opt->table.p || !opt->table.T || !(opt->table.h > 0)))
+        return fail(VAH_ERR_ARG, "vah_step_noalloc: EOS table needs n >= 2 nodes, h > 0 and device p, T arrays");
+    if (opt->eos != VAH_EOS_CONFORMAL && opt->eos != VAH_EOS_TABLE && opt->eos != VAH_EOS_WB)
+        return fail(VAH_ERR_ARG, "vah_step_noalloc: unknown EOS kind %d", opt->eos);
+    const vah_host::Layout lay(lat->nx, lat->ny, lat->nz);
+    const size_t need = (size_t)kStageFields * (size_t)lay.fs + 64;  // + 64 doubles of counters
+    if (need > kScratchDoubles)
+        return fail(VAH_ERR_ARG, "vah_step_noalloc: %ld padded cells per field exceed the scratch limit of %ld "
+                    "(%zu MiB for %d fields); use a context (vah_create)", lay.fs,
+                    (long)((kScratchDoubles - 64) / kStageFields), (size_t)(VAH_NOALLOC_SCRATCH_BYTES >> 20),
+                    kStageFields);
+    int dev = 0;
+    HIP_TRY(hipSetDevice(opt->device));
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(VAH_ERR_DEVICE, "vah_step_noalloc: device %d out of range", dev);
+    NoallocDevice *D = nullptr;
+    int rc = noalloc_setup(dev, &D);
+    if (rc != VAH_OK) return rc;
+    if (nsteps == 0) return VAH_OK;
+
+    // the EOS as the stage kernels read it: table arrays are the caller's device
+    // memory, the conformal bitmap is the static copy
+    Eos eos;
+    memset(&eos, 0, sizeof eos);
+    eos.kind = opt->eos;
+    if (opt->eos == VAH_EOS_CONFORMAL) {
+        eos.powf_fix = D->powf_fix;
+    } else if (opt->eos == VAH_EOS_TABLE) {
+        eos.n = opt->table.n;
+        eos.lne0 = opt->table.lne0;
+        eos.inv_h = 1.0 / opt->table.h;
+        eos.e_first = exp(opt->table.lne0);
+        eos.e_last = exp(opt->table.lne0 + (opt->table.n - 1) / (1.0 / opt->table.h));
+        eos.p = opt->table.p;
+        eos.T = opt->table.T;
+    }
+    double *scr = D->scratch;
+    unsigned int *cnt = counters ? counters : (unsigned int *)(scr + (size_t)kStageFields * lay.fs);
+    hipStream_t stream = (hipStream_t)opt->stream;
+    if (!counters) {
+        // a fill kernel (no memset node when the caller captures the stream)
+        hipLaunchKernelGGL(clear_counters_kernel, dim3(1), dim3(64), 0, stream, cnt);
+        HIP_TRY(hipGetLastError());
+    }
+    // zero-gradient ghosts of the inputs (every later input is a stage output,
+    // whose ghosts the stage kernels write)
+    {
+        const long n = lay.interior();
+        hipLaunchKernelGGL(ghost_fill_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, st->q, st->e,
+                           st->p, st->u, lay.fs, lat->nx, lat->ny, lat->nz, (long)lay.ncx,
+                           (long)lay.ncx * lay.ncy, lat->nz > 1 ? 1 : 0);
+        HIP_TRY(hipGetLastError());
+    }
+    const Geometry g = make_geometry(*lat, opt->eta_chunk);
+    const LaunchCfg L{g.grid, stream, opt->precision, opt->flux_mode, lat->nz > 1};
+    StageParams P = base_params(*lat, *hydro, lay, g, eos, cnt);
+    StepBuffers B;
+    B.q_star = scr;
+    B.e_star = scr + (size_t)NQ * lay.fs;
+    B.p_star = scr + (size_t)(NQ + 1) * lay.fs;
+    B.u_star = scr + (size_t)(NQ + 2) * lay.fs;
+    for (int it = 0; it < nsteps; ++it, ++nstep) {
+        // tau_n = t0 + n dt, as vah_step and the reference's driver compute it
+        const double t = hydro->t0 + (double)nstep * lat->dt;
+        B.q_n = st->q;
+        B.q_next = st->q_next;
+        B.e_n = st->e;
+        B.p_n = st->p;
+        B.e_next = st->e;  // stage 2 reads e*, p* only: e, p are updated in place
+        B.p_next = st->p;
+        B.u_n = st->u;
+        B.u_prev = st->u_prev;
+        B.u_next = st->u_prev;  // u^{n-1} is dead after stage 1
+        for (int stage = 1; stage <= 2; ++stage) {
+            set_stage_buffers(P, B, stage, t, lat->dt);
+            launch_stage(L, stage, P);
+            HIP_TRY(hipGetLastError());
+        }
+        // the reference's swaps: q <-> Q (HydroPlugin.cpp:230), up <-> u
+        std::swap(st->q, st->q_next);
+        std::swap(st->u, st->u_prev);
+    }
     return VAH_OK;
 }
 

@@ -168,6 +168,60 @@ int vah_reduce(vah_ctx *ctx, vah_stats *out);
  * vah_set_state / vah_set_primitives (VAH_ERR_STATE otherwise). */
 int vah_validity(vah_ctx *ctx, double *kn_T, double *kn_L, double *kn, double *r_pi, double *r_pi2);
 
+/* ---- Device-resident state without a context --------------------------- */
+
+/* Padded device layout of one field: (nx+4) x (ny+4) x (nz > 1 ? nz+4 : 1)
+ * cells, x fastest, 2 ghost cells per side (a boost-invariant lattice stores
+ * its one eta plane), rounded up to a multiple of 64 doubles; field f of a
+ * multi-field buffer starts f * VAH_FIELD_STRIDE doubles in.  Interior cell
+ * (i, j, k) sits at (k' * (ny+4) + j + 2) * (nx+4) + i + 2 with k' = k + 2 in
+ * 3+1D and 0 in 2+1D (old_cpu-vah_code/src/DynamicalVariables.cpp:29-31). */
+#define VAH_FIELD_STRIDE(nx, ny, nz)                                                            \
+    (((long long)((nx) + 4) * ((ny) + 4) * ((nz) > 1 ? (nz) + 4 : 1) + 63) / 64 * 64)
+
+/* Caller-owned device buffers of one lattice in that layout. */
+typedef struct vah_device_state {
+    double *q;        /* 15 fields: q^n in; after the call, the current level */
+    double *q_next;   /* 15 fields written with q^{n+1} (roles swap every step) */
+    double *e, *p;    /* level n in, the current level out (updated in place) */
+    double *u;        /* 4 fields u^tau, u^x, u^y, u^eta of level n; current level out */
+    double *u_prev;   /* 4 fields of level n-1 (the first call: equal to u); level n out */
+} vah_device_state;
+
+/* Bytes of the library's static device scratch array of vah_step_noalloc:
+ * the intermediate stage of 21 padded fields (q*, e*, p*, u*) and 64 doubles
+ * of counters must fit, i.e. VAH_FIELD_STRIDE(nx, ny, nz) <= 399,454 padded
+ * cells (every boost-invariant lattice up to 628 x 628; 3+1D lattices such
+ * as 101 x 101 x 30). */
+#define VAH_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+
+/* nsteps Heun-RK2 steps of a state the caller keeps in device memory,
+ * allocating nothing: no hipMalloc, hipFree or allocating library routine.
+ * No counterpart in the reference; it is for callers that cannot allocate --
+ * code under HIP graph capture, tight serving loops, harnesses that forbid
+ * it.  Same arithmetic as vah_step (bit-identical fields).
+ *  - lat, hydro, opt as for vah_create; opt->stream (NULL: the null stream)
+ *    receives all work and the call returns without waiting.  With
+ *    opt->eos == VAH_EOS_TABLE, opt->table.p and opt->table.T must be device
+ *    pointers (read during the steps).
+ *  - nstep: the step count of the input state (tau_n = t0 + nstep dt).
+ *  - st: the state; the ghost cells of q, e, p, u are filled first (zero
+ *    gradient, written in place), interior cells are only read.  On return
+ *    the pointers are rotated as the reference swaps q <-> Q and u <-> up:
+ *    st->q / st->u name the new level, st->q_next / st->u_prev the old one.
+ *  - counters: NULL, or 2+ device unsigned ints to which the NaN and
+ *    recovery-fallback counts are added (the caller clears them).  With
+ *    NULL they go to the scratch array, cleared by a kernel at each call.
+ * The intermediate stage lives in the library's static array of
+ * VAH_NOALLOC_SCRATCH_BYTES per GPU (kept until the process ends; its device
+ * address is looked up on the first call on a device, which also uploads the
+ * conformal EOS's 4 MB powf bitmap to a second static array and waits for
+ * it -- make that first call outside graph capture).  Every call on a GPU
+ * shares that GPU's arrays: calls that overlap on one GPU (other streams,
+ * other threads) race.  A lattice over the limit returns VAH_ERR_ARG. */
+int vah_step_noalloc(const vah_lattice *lat, const vah_hydro *hydro, const vah_options *opt, long long nstep,
+                     int nsteps, vah_device_state *st, unsigned int *counters);
+
 /* Per-launch HIP-event timing of the two stage kernels (off by default); the
  * reference times each step with a host stopwatch (rhic/src/HydroPlugin.cpp:222-227,234). */
 int vah_profile_enable(vah_ctx *ctx, int on);

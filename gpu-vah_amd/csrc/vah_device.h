@@ -316,6 +316,10 @@ struct Prim {
 };
 
 // Solve u_T/sqrt(1+u_T^2) (bL + b Pt) - bT = 0 (EnergyMomentumTensor.cpp:30-72).
+// The loop body has no branches: each pass evaluates the update and then
+// decides, as the reference does, whether it failed (e < 0 at the current
+// u_T, where the reference stops before the update, or an update that is NaN,
+// infinite, negative or above 9.0072e15), converged, or continues.
 __device__ __forceinline__ bool newton_uT(double t, double uT0, double MB0, double MBT, double MB3, double PL,
                                           double Ft, double x, double &uT) {
     const double Ft2 = Ft * Ft;
@@ -323,26 +327,27 @@ __device__ __forceinline__ bool newton_uT(double t, double uT0, double MB0, doub
     const double bL = x * x * MB0 - Ft2 * PL;
     const double b = x * x + Ft2;
     const double c0 = MB0 - t * Ft * MB3;
-    for (int it = 0; it < 10000; ++it) {
+    int st = 0;
+    for (int it = 0; it < 10000 && st == 0; ++it) {
         double F = 1.0 + uT0 * uT0;
         double rs = rsqrt(F);                  // 1/sqrt(1+u^2)
         double U = uT0 * rs;
         double e = c0 - U * x * MBT;
-        if (e < 0.0) return false;
         double Pt = 0.5 * (e - PL);
         double iP = rs * rs * rs;              // (1+u^2)^-1.5
         double g = bL + b * Pt;
         double dPtduT = -0.5 * x * MBT * iP;
         double f = U * g - bT;
         double fp = iP * g + U * b * dPtduT;
-        if (fabs(fp) == 0.0) fp = 1.e-16;
+        fp = fabs(fp) == 0.0 ? 1.e-16 : fp;
         double u1 = uT0 - fdiv(f, fp);
-        if (__builtin_isnan(u1) || __builtin_isinf(u1) || u1 < 0.0 || u1 > 9.0072e+15) return false;
-        uT = u1;
-        if (fabs(u1 - uT0) <= 1.e-7 * fabs(u1)) return true;
+        const bool bad = (e < 0.0) | __builtin_isnan(u1) | __builtin_isinf(u1) | (u1 < 0.0) | (u1 > 9.0072e+15);
+        const bool conv = fabs(u1 - uT0) <= 1.e-7 * fabs(u1);
+        st = bad ? -1 : (int)conv;
         uT0 = u1;
     }
-    return false;
+    uT = uT0;
+    return st == 1;
 }
 
 // getInferredVariables with W = Pi = 0 (EnergyMomentumTensor.cpp:74-197):

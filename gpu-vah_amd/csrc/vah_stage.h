@@ -104,8 +104,12 @@ __device__ __forceinline__ FaceState face_state(double t, double ttt, double ttx
     return f;
 }
 
-// One Newton update of u_T for state S from u0 (EnergyMomentumTensor.cpp:45-70).
-// Returns 1 converged, -1 failed, 0 continue; *u1 receives the update.
+// One Newton update of u_T for state S from u0 (EnergyMomentumTensor.cpp:45-70),
+// evaluated without branches: *u1 receives the update; the status is 1
+// converged, -1 failed (e < 0 at u0, or an update that is NaN, infinite,
+// negative or above 9.0072e15), 0 continue.  The reference stops before the
+// update when e < 0; the update computed here is then discarded, so the
+// decisions and the values kept are the reference's.
 __device__ __forceinline__ int newton_update(double t, const FaceState &S, double u0, double *u1) {
     const double Ft2 = S.Ft * S.Ft;
     const double xM = S.x * S.MBT;
@@ -115,36 +119,39 @@ __device__ __forceinline__ int newton_update(double t, const FaceState &S, doubl
     double rs = rsqrt(1.0 + u0 * u0);
     double U = u0 * rs;
     double e = c0 - U * xM;
-    if (e < 0.0) return -1;
     double iP = rs * rs * rs;
     double g = bL + b * 0.5 * (e - S.PL);
     double f = U * g - xM;
     double fp = iP * g - 0.5 * U * b * xM * iP;
-    if (fabs(fp) == 0.0) fp = 1.e-16;
+    fp = fabs(fp) == 0.0 ? 1.e-16 : fp;
     double u = u0 - fdiv(f, fp);
     *u1 = u;
-    if (__builtin_isnan(u) || __builtin_isinf(u) || u < 0.0 || u > 9.0072e+15) return -1;
-    return fabs(u - u0) <= 1.e-7 * fabs(u) ? 1 : 0;
+    // non-short-circuit operators: no branch splits the two sides' chains
+    const bool bad = (e < 0.0) | __builtin_isnan(u) | __builtin_isinf(u) | (u < 0.0) | (u > 9.0072e+15);
+    const bool conv = fabs(u - u0) <= 1.e-7 * fabs(u);
+    return bad ? -1 : (int)conv;
 }
 
 // Both sides of a face solved in one loop (independent iterations, each side
 // stops at its own convergence or failure, at most 10^4 iterations each).
-// Outputs the signed speed v = u^DIR / u^tau of each side; a failed recovery
-// gives the reference's fallback u = (1,0,0,0), i.e. v = 0.
+// Every pass evaluates both updates and keeps each only while its side is
+// still running, so the two dependency chains interleave and the loop body
+// has no branches.  Outputs the signed speed v = u^DIR / u^tau of each side;
+// a failed recovery gives the reference's fallback u = (1,0,0,0), i.e. v = 0.
 template <int DIR>
 __device__ __forceinline__ void face_speeds(double t, double guess, const FaceState &A, const FaceState &B, double &vA,
                                             double &vB) {
-    double a0 = guess, b0 = guess, ua = 0.0, ub = 0.0;
+    double ua = guess, ub = guess;
     int sa = 0, sb = 0;
     for (int it = 0; it < 10000 && (sa == 0 || sb == 0); ++it) {
-        if (sa == 0) {
-            sa = newton_update(t, A, a0, &ua);
-            a0 = ua;
-        }
-        if (sb == 0) {
-            sb = newton_update(t, B, b0, &ub);
-            b0 = ub;
-        }
+        double na, nb;
+        const int ta = newton_update(t, A, ua, &na);
+        const int tb = newton_update(t, B, ub, &nb);
+        const bool ra = sa == 0, rb = sb == 0;
+        ua = ra ? na : ua;
+        sa = ra ? ta : sa;
+        ub = rb ? nb : ub;
+        sb = rb ? tb : sb;
     }
     auto speed = [](const FaceState &S, double u, int st) {
         if (st != 1) return 0.0;

@@ -322,6 +322,59 @@ __device__ __forceinline__ void add_component(double &acc, int n, R Hm, R Hp, co
     acc += (double)r * dt;
 }
 
+// the flux part of one component: acc += dt * (H- - H+)/d (the difference in R,
+// the accumulation in double); the axis sources are added separately
+template <typename R>
+__device__ __forceinline__ void add_flux(double &acc, R Hm, R Hp, R inv_d, double dt) {
+    acc += (double)((Hm - Hp) * inv_d) * dt;
+}
+
+// acc^{tau mu} += dt * S_axis of every direction, after the source terms.  The
+// loads of all directions (fourteen each) are issued together, one wait for
+// all; the velocities u^i come from the source terms' own loads.
+template <bool D3, typename R>
+__device__ __forceinline__ void add_axis_sources(const StageParams &P, const Views &V, uint32_t s, const SrcIn &in,
+                                                 double iut, double (&acc)[NQ]) {
+    const uint32_t f8 = V.f8;
+    auto fields = [](int dir, int i) {
+        constexpr int T[4] = {PITT, PITX, PITY, PITN};
+        constexpr int X[3] = {PIXX, PIXY, PIXN}, Y[3] = {PIXY, PIYY, PIYN}, Z[3] = {PIXN, PIYN, PINN};
+        return i < 4 ? T[i] : dir == 1 ? X[i - 4] : dir == 2 ? Y[i - 4] : Z[i - 4];
+    };
+    constexpr int ND = D3 ? 3 : 2;
+    const uint32_t stp[3] = {8u, V.sy8, V.sz8};
+    double up[ND][7], dn[ND][7];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+#pragma unroll
+        for (int i = 0; i < 7; ++i) {
+            up[d][i] = ld(V.qin, s + stp[d], fields(d + 1, i) * f8);
+            dn[d][i] = ld(V.qin, s - stp[d], fields(d + 1, i) * f8);
+        }
+    }
+    __builtin_amdgcn_sched_barrier(0);  // keep the loads together (not sunk to their uses)
+    const double h[3] = {P.hx, P.hy, P.hz};
+    const double v[3] = {in.ux * iut, in.uy * iut, in.un * iut};
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+        auto dd = [&](int i) { return (R)((up[d][i] - dn[d][i]) * h[d]); };
+        AxisPi<R> a;
+        a.dtt = dd(0);
+        a.dtx = dd(1);
+        a.dty = dd(2);
+        a.dtn = dd(3);
+        a.dix = dd(4);
+        a.diy = dd(5);
+        a.din = dd(6);
+        R Sa[4];
+        if (d == 0) axis_source<1>(a, (R)v[0], Sa);
+        else if (d == 1) axis_source<2>(a, (R)v[1], Sa);
+        else axis_source<3>(a, (R)v[2], Sa);
+#pragma unroll
+        for (int n = 0; n < 4; ++n) acc[n] += (double)Sa[n] * P.dt;
+    }
+}
+
 template <bool D3>
 __device__ __forceinline__ void load_src_in(const StageParams &P, const Views &V, uint32_t s, SrcIn &in) {
     const Buf &u = V.ucur;
@@ -498,6 +551,8 @@ __device__ __forceinline__ void stage_tile(const StageParams &P, int tile, R (*H
             source_terms<R>(P.eos, in, Q, S, nan);
 #pragma unroll
             for (int n = 0; n < NQ; ++n) acc[n] = Q[n] + dt * (double)S[n];
+            iut = rcp(in.ut);
+            if (SHARED) add_axis_sources<D3, R>(P, V, s, in, iut, acc);
             // the cell recovery's inputs come from the same loads: e at the
             // stage's input level (its fallback base) and |u_T| of the velocity
             // that seeds it, u_cur in stage 1 and u_prev in stage 2
@@ -505,7 +560,6 @@ __device__ __forceinline__ void stage_tile(const StageParams &P, int tile, R (*H
             rec.e_prev = in.e;
             rec.uT_guess = STAGE2 ? sqrt_pos(fabs(in.uxp * in.uxp + in.uyp * in.uyp))
                                   : sqrt_pos(fabs(in.ux * in.ux + in.uy * in.uy));
-            iut = rcp(in.ut);
         }
 
         if (SHARED) {
@@ -522,10 +576,8 @@ __device__ __forceinline__ void stage_tile(const StageParams &P, int tile, R (*H
             __syncthreads();
             if (cell_ok) {
                 const int fm = cj * (TX + 1) + ci;
-                R Sa[4];
-                axis_source<1>(axis_pi<1, R>(V.qin, f8, s, 8, P.hx), (R)(ld(V.ucur, s, f8) * iut), Sa);
 #pragma unroll
-                for (int n = 0; n < NQ; ++n) add_component(acc[n], n, Hs[n][fm], Hs[n][fm + 1], Sa, inv_dx, dt);
+                for (int n = 0; n < NQ; ++n) add_flux(acc[n], Hs[n][fm], Hs[n][fm + 1], inv_dx, dt);
             }
             __syncthreads();
             // ---- y faces: TX x (TY+1), face f between cells (fi, fj-1) and (fi, fj)
@@ -541,10 +593,8 @@ __device__ __forceinline__ void stage_tile(const StageParams &P, int tile, R (*H
             __syncthreads();
             if (cell_ok) {
                 const int fm = cj * TX + ci;
-                R Sa[4];
-                axis_source<2>(axis_pi<2, R>(V.qin, f8, s, sy8, P.hy), (R)(ld(V.ucur, s, 2 * f8) * iut), Sa);
 #pragma unroll
-                for (int n = 0; n < NQ; ++n) add_component(acc[n], n, Hs[n][fm], Hs[n][fm + TX], Sa, inv_dy, dt);
+                for (int n = 0; n < NQ; ++n) add_flux(acc[n], Hs[n][fm], Hs[n][fm + TX], inv_dy, dt);
             }
             __syncthreads();
             // ---- eta faces: solved by the cell's own lane, the upper one carried to the next plane
@@ -571,10 +621,8 @@ __device__ __forceinline__ void stage_tile(const StageParams &P, int tile, R (*H
                                         [&](int n, R h) { Hz[n][tid] = h; });
                     }
                 }
-                R Sa[4];
-                axis_source<3>(axis_pi<3, R>(V.qin, f8, s, sz8, P.hz), (R)(ld(V.ucur, s, 3 * f8) * iut), Sa);
                 auto upper = [&](int n, R h) {
-                    add_component(acc[n], n, Hz[n][tid], h, Sa, inv_dz, dt);
+                    add_flux(acc[n], Hz[n][tid], h, inv_dz, dt);
                     Hz[n][tid] = h;
                 };
                 if constexpr (ZRING) {

@@ -198,6 +198,9 @@ void fill_descriptor(const vah_batch *b, HostEvent &h) {
     P.hx = 1 / d.dx / 2;
     P.hy = 1 / d.dy / 2;
     P.hz = 1 / d.dz / 2;
+    P.dt_dx = P.dt * P.inv_dx;
+    P.dt_dy = P.dt * P.inv_dy;
+    P.dt_dz = P.dt * P.inv_dz;
     P.eos = b->eos;
     double *base = (double *)(b->arena + h.offset);
     const long fs = h.lay.fs;

@@ -223,6 +223,9 @@ StageParams base_params(const vah_lattice &lat, const vah_hydro &hydro, const va
     P.hx = 1 / lat.dx / 2;
     P.hy = 1 / lat.dy / 2;
     P.hz = 1 / lat.dz / 2;
+    P.dt_dx = P.dt * P.inv_dx;
+    P.dt_dy = P.dt * P.inv_dy;
+    P.dt_dz = P.dt * P.inv_dz;
     P.eos = eos;
     P.counters = counters;
     P.emax = nullptr;

@@ -307,6 +307,29 @@ __device__ __forceinline__ float half_slope(float x, float y, float z) {
     return __builtin_amdgcn_fmed3f(l, __builtin_amdgcn_fmed3f(c, r, 0.0f), 0.0f);
 }
 
+// Four times half_slope, for a face value formed as fma(slope4, +-1/4, q):
+// (theta / 2) m scaled by 4 is 2.2 m to the bit (2.2 as a double is 4 times
+// 0.55 as a double, and scaling by a power of two commutes with rounding), and
+// |z - x| / 4 scaled by 4 is |z - x|, so min(2.2 min(|a|, |b|), |z - x|) is 4 x
+// the half slope exactly, and q + s4 / 4 rounds once, as q + half_slope does.
+// One instruction fewer per slope: the quarter rides on the face value's add.
+__device__ __forceinline__ double slope4(double x, double y, double z) {
+    constexpr double kTwiceTheta = 1.1 * 2.0;
+    const double a = y - x, b = z - y;
+    const double m = fmin(kTwiceTheta * fmin(fabs(a), fabs(b)), fabs(z - x));
+    return a * b > 0.0 ? __builtin_copysign(m, a) : 0.0;
+}
+
+// left and right face values of the stencil x0..x3 (faces of x1 | x2)
+__device__ __forceinline__ void face_values(double q0, double q1, double q2, double q3, double &l, double &r) {
+    l = __builtin_fma(slope4(q0, q1, q2), 0.25, q1);
+    r = __builtin_fma(slope4(q1, q2, q3), -0.25, q2);
+}
+__device__ __forceinline__ void face_values(float q0, float q1, float q2, float q3, float &l, float &r) {
+    l = q1 + half_slope(q0, q1, q2);
+    r = q2 - half_slope(q1, q2, q3);
+}
+
 // ---------------------------------------------------------------------------
 // Newton recovery of (e, p, u^mu) from T^tau-mu, P_L, pi^tau-mu.
 // ---------------------------------------------------------------------------

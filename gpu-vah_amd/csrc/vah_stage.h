@@ -65,6 +65,7 @@ struct StageParams {
     double dt, dt_inv, etabar, etabar_inv;
     double inv_dx, inv_dy, inv_dz;  // 1/d
     double hx, hy, hz;              // (1/d)/2, the reference's centred-difference factor
+    double dt_dx, dt_dy, dt_dz;     // dt * (1/d), the weight of a flux difference
     Eos eos;
     unsigned int *counters;  // [0] NaN cells, [1] recovery fallbacks
     unsigned long long *emax;  // running max of the new e (ordered bits), or null
@@ -241,8 +242,7 @@ __device__ __forceinline__ void face_flux_from(double t, Guess guess, Load load,
 #pragma unroll
     for (int n = 0; n < NN; ++n) {
         R q0 = (R)raw[n][0], q1 = (R)raw[n][1], q2 = (R)raw[n][2], q3 = (R)raw[n][3];
-        qL[n] = q1 + half_slope(q0, q1, q2);
-        qR[n] = q2 - half_slope(q1, q2, q3);
+        face_values(q0, q1, q2, q3, qL[n], qR[n]);
     }
     // the Newton recovery of both sides runs in double whatever R is
     FaceState FR = face_state(t, qR[TTT], qR[TTX], qR[TTY], qR[TTN], qR[PL], qR[PITT], qR[PITX], qR[PITY], qR[PITN]);
@@ -263,8 +263,8 @@ __device__ __forceinline__ void face_flux_from(double t, Guess guess, Load load,
 #pragma unroll
     for (int n = NN; n < NQ; ++n) {
         R q0 = (R)rest[n - NN][0], q1 = (R)rest[n - NN][1], q2 = (R)rest[n - NN][2], q3 = (R)rest[n - NN][3];
-        R l = q1 + half_slope(q0, q1, q2);
-        R r = q2 - half_slope(q1, q2, q3);
+        R l, r;
+        face_values(q0, q1, q2, q3, l, r);
         emit(n, cR * r + cL * l);
     }
 }
@@ -322,10 +322,15 @@ __device__ __forceinline__ void add_component(double &acc, int n, R Hm, R Hp, co
     acc += (double)r * dt;
 }
 
-// the flux part of one component: acc += dt * (H- - H+)/d (the difference in R,
-// the accumulation in double); the axis sources are added separately
-template <typename R>
-__device__ __forceinline__ void add_flux(double &acc, R Hm, R Hp, R inv_d, double dt) {
+// the flux part of one component: acc += dt * (H- - H+)/d, the axis sources
+// being added separately.  In double one fused multiply-add with dt_d = dt/d
+// (precomputed with the stage parameters); in float the difference is formed
+// in float and accumulated in double.
+__device__ __forceinline__ void add_flux(double &acc, double Hm, double Hp, double /*inv_d*/, double dt_d,
+                                         double /*dt*/) {
+    acc = __builtin_fma(Hm - Hp, dt_d, acc);
+}
+__device__ __forceinline__ void add_flux(double &acc, float Hm, float Hp, float inv_d, double /*dt_d*/, double dt) {
     acc += (double)((Hm - Hp) * inv_d) * dt;
 }
 
@@ -577,7 +582,7 @@ __device__ __forceinline__ void stage_tile(const StageParams &P, int tile, R (*H
             if (cell_ok) {
                 const int fm = cj * (TX + 1) + ci;
 #pragma unroll
-                for (int n = 0; n < NQ; ++n) add_flux(acc[n], Hs[n][fm], Hs[n][fm + 1], inv_dx, dt);
+                for (int n = 0; n < NQ; ++n) add_flux(acc[n], Hs[n][fm], Hs[n][fm + 1], inv_dx, P.dt_dx, dt);
             }
             __syncthreads();
             // ---- y faces: TX x (TY+1), face f between cells (fi, fj-1) and (fi, fj)
@@ -594,7 +599,7 @@ __device__ __forceinline__ void stage_tile(const StageParams &P, int tile, R (*H
             if (cell_ok) {
                 const int fm = cj * TX + ci;
 #pragma unroll
-                for (int n = 0; n < NQ; ++n) add_flux(acc[n], Hs[n][fm], Hs[n][fm + TX], inv_dy, dt);
+                for (int n = 0; n < NQ; ++n) add_flux(acc[n], Hs[n][fm], Hs[n][fm + TX], inv_dy, P.dt_dy, dt);
             }
             __syncthreads();
             // ---- eta faces: solved by the cell's own lane, the upper one carried to the next plane
@@ -622,7 +627,7 @@ __device__ __forceinline__ void stage_tile(const StageParams &P, int tile, R (*H
                     }
                 }
                 auto upper = [&](int n, R h) {
-                    add_flux(acc[n], Hz[n][tid], h, inv_dz, dt);
+                    add_flux(acc[n], Hz[n][tid], h, inv_dz, P.dt_dz, dt);
                     Hz[n][tid] = h;
                 };
                 if constexpr (ZRING) {

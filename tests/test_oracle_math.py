@@ -157,6 +157,29 @@ def test_device_half_slope_is_half_the_reference_slope(oracle_mod):
     assert np.all(got == 0.0) and np.all(np.abs(want) < 2.4e-162) and np.all(want != 0.0)
 
 
+def _device_slope4(x, y, z):
+    """slope4 of gpu-vah_amd/csrc/vah_device.h: four times the half slope, with
+    2.2 = 4 x 0.55 and |z - x| unscaled; the face value is fma(s4, 1/4, q)."""
+    a, b = y - x, z - y
+    m = np.minimum((1.1 * 2.0) * np.minimum(np.abs(a), np.abs(b)), np.abs(z - x))
+    return np.where(a * b > 0, np.copysign(m, a), 0.0)
+
+
+def test_device_slope4_is_four_half_slopes_to_the_bit():
+    """The double kernels form the face values as fma(slope4, +-1/4, q): a
+    quarter of slope4 must be the half slope bit for bit (scaling by a power of
+    two commutes with rounding, and 2.2 as a double is 4 x 0.55 as a double), so
+    that q +- slope4 / 4, rounded once by the fma, is q +- half_slope."""
+    assert 1.1 * 2.0 == 4.0 * (1.1 * 0.5)
+    rng = np.random.default_rng(13)
+    x, y, z = _scaled_triples(rng, 20000, -150, 150, np.float64)
+    s4 = _device_slope4(x, y, z)
+    hs = _device_half_slope(x, y, z)
+    assert np.array_equal(0.25 * s4, hs)
+    assert np.array_equal(y + 0.25 * s4, y + hs) and np.array_equal(z - 0.25 * _device_slope4(y, z, x),
+                                                                    z - _device_half_slope(y, z, x))
+
+
 def test_float_half_slope_median_form():
     """The split-precision kernels' float slope, med3(l, med3(c, r, 0), 0),
     against the sign-test form min(theta/2 min(|a|, |b|), |z - x| / 4) with the

@@ -223,14 +223,33 @@ void drop_graphs(vah_batch *b) {
 // uploads are ordered after the work that reads the old lists)
 int upload_lists(vah_batch *b) {
     std::vector<int2> t2, t3;
-    std::vector<int> act;
+    std::vector<int> act, ev3;
     for (int s = 0; s < b->max_events; ++s) {
         const HostEvent &h = b->events[s];
         if (!h.used || h.stopped) continue;  // a stopped event's fields stay put without its tiles
         act.push_back(s);
+        if (h.desc.nz > 1) {
+            ev3.push_back(s);
+            continue;
+        }
         const int nt = tiles_of(b, h);
-        std::vector<int2> &dst = h.desc.nz > 1 ? t3 : t2;
-        for (int t = 0; t < nt; ++t) dst.push_back(make_int2(s, t));
+        for (int t = 0; t < nt; ++t) t2.push_back(make_int2(s, t));
+    }
+    // 3+1D tiles: the kernel gives XCD x the x-th eighth of the list, in list
+    // order.  The list is therefore eight groups, group x holding the x-th
+    // eighth of every eta chunk's tiles, chunk after chunk: the XCDs work
+    // through each event's chunks together, so that the planes in flight form
+    // one band, as in the single-lattice kernel's workgroup order.
+    const vah_batch_variant &v = kVariants[b->variant].info;
+    for (int x = 0; x < 8; ++x) {
+        for (int s : ev3) {
+            const HostEvent &h = b->events[s];
+            const int tpl = ((h.desc.nx + v.tile_x - 1) / v.tile_x) * ((h.desc.ny + v.tile_y - 1) / v.tile_y);
+            const int zc = tiles_of(b, h) / tpl;
+            const int t0 = tpl * x / 8, t1 = tpl * (x + 1) / 8;
+            for (int c = 0; c < zc; ++c)
+                for (int t = t0; t < t1; ++t) t3.push_back(make_int2(s, c * tpl + t));
+        }
     }
     if (t2.size() > b->cap2 || t3.size() > b->cap3) {
         HIP_TRY(hipStreamSynchronize(b->stream));

@@ -192,7 +192,7 @@ Geometry make_geometry(const vah_lattice &lat, int eta_chunk) {
     g.kz = lat.nz > 1 ? std::min(eta_chunk > 0 ? eta_chunk : kz_default, lat.nz) : 1;
     g.zchunks = (lat.nz + g.kz - 1) / g.kz;
     g.nblocks = g.tiles_x * g.tiles_y * g.zchunks;
-    g.grid = (g.nblocks + 7) / 8 * 8;
+    g.grid = g.zchunks * 8 * ((g.tiles_x * g.tiles_y + 7) / 8);  // tile_of_block's grid
     return g;
 }
 

@@ -17,9 +17,8 @@
 // carried to the next plane in an LDS slot only that lane touches (in
 // registers it would push the kernel into spilling).  Without SHARED every cell solves
 // both of its faces with its own Newton guess, the reference's work split.
-// blockIdx is remapped so that consecutive tiles land on the same XCD (the
-// hardware deals workgroups round-robin over the 8 XCDs), keeping tile halos
-// in one L2.
+// Workgroups are mapped to tiles so that the eight XCDs work through the eta
+// chunks together (tile_of_block).
 #pragma once
 
 #include "vah_device.h"
@@ -60,7 +59,7 @@ struct StageParams {
     int ncx, ncy;         // computational extents (interior + 4)
     long sy, sz;          // strides in y and eta
     int tiles_x, tiles_y, zchunks, kz;
-    int nblocks;          // real workgroups (the grid is padded to a multiple of 8)
+    int nblocks;          // tiles of the lattice (tiles_x * tiles_y * zchunks)
     double t_src, t_rec;  // tau of the sources/fluxes, tau of the recovery
     double dt, dt_inv, etabar, etabar_inv;
     double inv_dx, inv_dy, inv_dz;  // 1/d
@@ -71,10 +70,20 @@ struct StageParams {
     unsigned long long *emax;  // running max of the new e (ordered bits), or null
 };
 
-__device__ __forceinline__ int remap_block(int b, int nb) {
-    // workgroup b runs on XCD b % 8; give each XCD a contiguous run of tiles
-    int per = nb >> 3;
-    return (b & 7) * per + (b >> 3);
+// Workgroup -> tile.  The hardware deals workgroups round-robin over the
+// eight XCDs (workgroup b runs on XCD b % 8).  All XCDs work through the eta
+// chunks together, each on its own slice of a chunk's tiles: XCD x takes the
+// tiles x * slice .. (x + 1) * slice - 1 of chunk 0, then of chunk 1, and so
+// on.  The planes in flight are then those of a few neighbouring chunks, one
+// band that the memory-side cache can hold, and a tile's eta neighbour in the
+// next chunk runs on the same XCD shortly after it; giving each XCD a
+// contiguous run of all tiles instead keeps eight bands in flight, one per
+// XCD (7 % slower at 201x201x101).  The grid holds zchunks * 8 * slice
+// workgroups; those past the end of a chunk's tiles get -1.
+__device__ __forceinline__ int tile_of_block(const StageParams &P, int b) {
+    const int tpl = P.tiles_x * P.tiles_y, slice = (tpl + 7) >> 3;
+    const int l = b >> 3, c = l / slice, within = (b & 7) * slice + (l - c * slice);
+    return within < tpl && c < P.zchunks ? c * tpl + within : -1;
 }
 
 // ---------------------------------------------------------------------------
@@ -698,9 +707,8 @@ __global__ __launch_bounds__(NTHREADS, D3 ? VAH_BLOCKS_3D : VAH_BLOCKS_2D) void 
     const StageParams &P = *(const StageParams *)(KernargParams *)__builtin_amdgcn_kernarg_segment_ptr();
     (void)P_arg;
     __shared__ StageLds<D3, SHARED, R> lds;
-    const int b = (int)blockIdx.x;
-    const int tile = (gridDim.x & 7) == 0 ? remap_block(b, (int)gridDim.x) : b;
-    if (tile >= P.nblocks) return;  // padding workgroups: no barrier reached
+    const int tile = tile_of_block(P, (int)blockIdx.x);
+    if (tile < 0) return;  // padding workgroups: no barrier reached
     stage_tile<D3, STAGE2, SHARED, R>(P, tile, lds.Hs, lds.Hz, &lds.Zr[0][0][0]);
 }
 

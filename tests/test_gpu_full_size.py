@@ -3,7 +3,9 @@
 dtau = 0.02, eta/s = 0.16, the 10^4-node LQCD-like EOS table).
 
 - A few steps against the CPU oracle on the same input, cell by cell: FP64
-  within 1e-10 (metric in tests/parity.py), split precision within 1e-4.
+  within 1e-10 (metric in tests/parity.py), split precision within 1e-4; and
+  a few steps from the developed flow of step 60, handed from the GPU to the
+  oracle.
 - Boost invariance at that size, a property that needs no oracle: an
   eta-independent initial state stays eta-independent, every eta plane bit for
   bit (each plane runs the same arithmetic, whichever eta chunk and ring slot
@@ -14,7 +16,7 @@ dtau = 0.02, eta/s = 0.16, the 10^4-node LQCD-like EOS table).
 import numpy as np
 import pytest
 
-from tests.parity import assert_parity, vah
+from tests.parity import assert_parity, field_errors, vah
 
 pytestmark = pytest.mark.gpu
 
@@ -69,6 +71,50 @@ def test_201x201x101_against_oracle(oracle_mod):
     err_u = max(float(np.max(np.abs(s["u"][d] - b["u"][d]) / b["u"][0])) for d in range(4))
     print("split max rel error e", err_e, "u", err_u, flush=True)
     assert err_e < 1e-4 and err_u < 1e-4
+
+
+def test_201x201x101_developed_flow_against_oracle(oracle_mod):
+    """The full-size event after 60 GPU steps (tau = 1.7 fm/c, transverse
+    velocity up to 0.8 at the fireball edge), handed to the oracle with its
+    previous velocity: 3 more steps on both agree per cell within
+    max(1e-10, 10 s), s the oracle's own spread under a 1e-15 relative
+    perturbation of that state."""
+    V = vah()
+    table = V.eos_lqcd_table(10000, 1e-8, 1e4)
+    e = mc_glauber(V, NZ)
+    n = e.size
+    print("201x201x101 MC-Glauber, 60 GPU steps, then 3 steps on both", flush=True)
+    g = V.Vah(NX, NY, NZ, DX, DX, DZ, DT, ETABAR, T0, eos_table=table)
+    g.set_primitives(e, *rest(n), pl=e / 3)
+    g.step(60)
+    g.synchronize()
+    st = g.state()
+    vT = float(np.max(np.hypot(st["u"][1], st["u"][2]) / st["u"][0]))
+    print(f"max transverse velocity at step 60: {vT:.3f}", flush=True)
+    assert vT > 0.5  # developed transverse flow (0.81 at step 60)
+    g.step(3)
+    g.synchronize()
+    a, tau_g = g.state(), g.time
+    g.close()
+
+    def oracle_from(q):
+        o = oracle_mod.Oracle(NX, NY, NZ, DX, DX, DZ, DT, ETABAR, T0, eos_table=table)
+        o.set_state(60, q, st["e"], st["p"], st["u"], st["up"])
+        o.step(3)
+        b, tau, fails = o.state(), o.time, o.newton_fail
+        o.close()
+        return b, tau, fails
+
+    b, tau, fails = oracle_from(st["q"])
+    assert tau == tau_g and fails == 0
+    rng = np.random.default_rng(5)
+    b2, _, _ = oracle_from(st["q"] * (1.0 + 1e-15 * rng.standard_normal(st["q"].shape)))
+    spread = max(field_errors(b2, b, tau).values())
+    bar = max(1e-10, 10 * spread)
+    err = field_errors(a, b, tau)
+    worst = max(err.items(), key=lambda kv: kv[1])
+    print(f"oracle self-spread {spread:.1e}; GPU vs oracle worst {worst[0]} {worst[1]:.1e} (bar {bar:.1e})", flush=True)
+    assert worst[1] <= bar
 
 
 def test_201x201x101_boost_invariance():

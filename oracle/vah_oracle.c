@@ -45,9 +45,11 @@
  * composition and ghost fill are pinned by round trips, closed forms, the
  * conformal Bjorken solution and lattice symmetries; the Wuppertal-Budapest
  * fits by thermodynamic consistency (T ds/de = 1).  orc_validity is pinned on
- * Bjorken flow (theta, the longitudinal projection A, the Knudsen numbers);
- * its u^eta-dependent terms of A and its R_pi with nonzero pi are parity
- * unpinned (the reference file including them needs libconfig to build).
+ * Bjorken and uniform longitudinal flow (theta, the longitudinal projection
+ * A, the Knudsen numbers), and every map -- Kn_T, Kn_L, Kn, R_pi, R_pi2 --
+ * against the covariant definitions evaluated independently in Milne
+ * coordinates (tests/milne.py) on states with transverse and longitudinal
+ * flow and a nonzero pi.
  *
  * The tabulated equation of state (EOS_TABLE) is not in the reference: it is
  * this project's lattice-QCD-like EOS, looked up exactly as the product does.

@@ -54,6 +54,31 @@ def test_3d_small(oracle_mod, flux_mode):
     assert_parity(a, b, tau, label="3D")
 
 
+@pytest.mark.parametrize("dims", [(1, 1, 1), (1, 7, 1), (5, 1, 3), (2, 2, 2), (3, 4, 1)],
+                         ids=["1x1x1", "1x7x1", "5x1x3", "2x2x2", "3x4x1"])
+def test_degenerate_lattices(oracle_mod, dims):
+    """Lattices one or two cells wide in some direction, down to a single
+    cell: a tile with a handful of live lanes, ghost copies that are the cell
+    itself, 20 steps against the oracle in both flux modes."""
+    nx, ny, nz = dims
+    e = gaussian_ic(nx, ny, nz, sx=0.3, sy=0.4, seed=9)
+    n = e.size
+    o = oracle_mod.Oracle(nx, ny, nz, 0.1, 0.1, 0.1, 0.01, 0.2, 0.5)
+    o.set_primitives(e, *rest_frame(n), pl=e / 3)
+    o.step(20)
+    b, tau = o.state(), o.time
+    o.close()
+    V = vah()
+    for mode in (V.FLUX_SHARED, V.FLUX_PER_CELL):
+        g = V.Vah(nx, ny, nz, 0.1, 0.1, 0.1, 0.01, 0.2, 0.5, flux_mode=mode)
+        g.set_primitives(e, *rest_frame(n), pl=e / 3)
+        g.step(20)
+        g.synchronize()
+        assert g.time == tau
+        assert_parity(g.state(), b, tau, label=f"{nx}x{ny}x{nz} mode {mode}")
+        g.close()
+
+
 @pytest.mark.parametrize("dims", [(17, 31, 2), (33, 16, 4), (16, 15, 7)], ids=["17x31x2", "33x16x4", "16x15x7"])
 def test_3d_ragged_tiles_and_eta_chunks(oracle_mod, dims):
     """3+1D lattices whose sizes are not multiples of the 16 x 15 tile or of the

@@ -103,6 +103,34 @@ def test_batch_matches_single_lattice_path_and_oracle(oracle_mod):
     B.close()
 
 
+def test_degenerate_events_in_a_batch(oracle_mod):
+    """Events one or two cells wide, down to a single cell, resident beside a
+    normal event: each within 1e-10 of the oracle run of the same event (the
+    batch's tile lists and persistent launches hold tiles with a handful of
+    live lanes)."""
+    V = vah()
+    shapes = [(1, 1, 1), (3, 4, 1), (5, 1, 3), (2, 2, 2), (40, 36, 1)]
+    evs = [dict(nx=nx, ny=ny, nz=nz, dx=0.1, dy=0.1, dz=0.1, dt=0.01, t0=0.5, etabar=0.2,
+                e=gaussian_ic(nx, ny, nz, sx=0.3 if nx * ny < 100 else 1.5, sy=0.4 if nx * ny < 100 else 1.8,
+                              seed=9 + i))
+           for i, (nx, ny, nz) in enumerate(shapes)]
+    steps = 20
+    B = V.Batch(budget_bytes=1 << 30, max_events=8, freezeout_T_GeV=0.0, steps_per_graph=4)
+    slots = [admit(B, ev, max_steps=steps, tag=i) for i, ev in enumerate(evs)]
+    done = run_until_stopped(B, slots, chunk=5)
+    for i, (ev, s) in enumerate(zip(evs, slots)):
+        assert done[s]["nstep"] == steps
+        n = ev["nx"] * ev["ny"] * ev["nz"]
+        o = oracle_mod.Oracle(ev["nx"], ev["ny"], ev["nz"], ev["dx"], ev["dy"], ev["dz"], ev["dt"], ev["etabar"],
+                              ev["t0"])
+        o.set_primitives(ev["e"], *rest_frame(n), pl=ev["e"] / 3)
+        o.step(steps)
+        assert done[s]["tau"] == o.time
+        assert_parity(B.state(s), o.state(), o.time, TOL_FP64, label=f"batch event {shapes[i]}")
+        o.close()
+    B.close()
+
+
 @pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "fp32_split"])
 def test_event_alone_equals_event_in_the_mix(precision):
     """One event run alone and run among others that are admitted and retired

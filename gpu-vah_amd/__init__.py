@@ -221,6 +221,7 @@ def lib():
         L.vah_config_load.argtypes = [ctypes.c_char_p, ctypes.POINTER(Config)]
         L.vah_ic_reference.argtypes = [ctypes.POINTER(Config)] + [_dp] * 5
         L.vah_run.argtypes = [ctypes.POINTER(Config), ctypes.c_char_p, ctypes.c_int]
+        L.vah_write_dat.argtypes = [ctypes.POINTER(Config), _dp, ctypes.c_double, ctypes.c_char_p, ctypes.c_char_p]
         L.vah_step_noalloc.argtypes = [ctypes.POINTER(Lattice), ctypes.POINTER(Hydro), ctypes.POINTER(Options),
                                        ctypes.c_longlong, ctypes.c_int, ctypes.POINTER(DeviceState),
                                        ctypes.c_void_p]
@@ -313,6 +314,12 @@ def config_load(directory):
     c = Config()
     _check(lib().vah_config_load(directory.encode(), ctypes.byref(c)))
     return c
+
+
+def write_dat(cfg, var, tau, directory, name):
+    """One interior field as <directory>/<name>_<tau>.dat in the reference's text format."""
+    v = np.ascontiguousarray(var, dtype=np.float64).ravel()
+    _check(lib().vah_write_dat(ctypes.byref(cfg), v.ctypes.data_as(_dp), tau, directory.encode(), name.encode()))
 
 
 def ic_reference(cfg):

@@ -10,9 +10,12 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
+#include <charconv>
 #include <chrono>
 #include <cstdlib>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/vah_host.h"
@@ -121,23 +124,76 @@ void get_double(const std::vector<Setting> &st, const char *name, double *v, dou
         if (s.name == name) *v = s.kind == FLOAT ? s.fval : dflt;
 }
 
-void write_dat(const double *var, double t, const char *dir, const char *name, const vah_config *c) {
+// Threads that format one output file (the box's CPU share: at most 16).
+int writer_threads() {
+    const unsigned h = std::thread::hardware_concurrency();
+    return (int)std::max(1u, std::min(16u, h));
+}
+
+// printf's "%.*f" of v appended to out: std::to_chars in fixed notation gives
+// the same correctly rounded digits as printf in the C locale (sign of zero,
+// nan and inf included); a value too long for the scratch span takes printf.
+inline void append_fixed(std::string &out, double v, int prec) {
+    char buf[64];
+    auto r = std::to_chars(buf, buf + sizeof buf, v, std::chars_format::fixed, prec);
+    if (r.ec == std::errc()) {
+        out.append(buf, r.ptr);
+        return;
+    }
+    char big[400];
+    int m = snprintf(big, sizeof big, "%.*f", prec, v);
+    out.append(big, (size_t)std::max(0, std::min(m, (int)sizeof big - 1)));
+}
+
+// The reference's output() (FileIO.cpp:14-59): one line
+// "%.3f\t%.3f\t%.3f\t%.8f\n" of x, y, eta and the value per interior cell, eta
+// outermost, x fastest.  The coordinate columns repeat row after row, so each
+// x column is formatted once and each row's "y\teta\t" once per row; the
+// values are formatted in parallel, row blocks per thread, and the blocks
+// written in order.
+int write_dat(const double *var, double t, const char *dir, const char *name, const vah_config *c) {
     char fname[1024];
     snprintf(fname, sizeof fname, "%s/%s_%.3f.dat", dir, name, t);
     FILE *fp = fopen(fname, "w");
-    if (!fp) return;
+    if (!fp) return VAH_ERR_ARG;
     const int nx = c->numLatticePointsX, ny = c->numLatticePointsY, nz = c->numLatticePointsRapidity;
-    for (int k = 0; k < nz; ++k) {
-        double z = (k - (nz - 1) / 2.) * c->latticeSpacingRapidity;
-        for (int j = 0; j < ny; ++j) {
-            double y = (j - (ny - 1) / 2.) * c->latticeSpacingY;
+    std::vector<std::string> xs(nx);
+    for (int i = 0; i < nx; ++i) {
+        append_fixed(xs[i], (i - (nx - 1) / 2.) * c->latticeSpacingX, 3);
+        xs[i].push_back('\t');
+    }
+    const long rows = (long)ny * nz;
+    const int nt = (int)std::min<long>(writer_threads(), rows);
+    std::vector<std::string> part(nt);
+    auto work = [&](int w) {
+        std::string &o = part[w], yz;
+        const long r0 = rows * w / nt, r1 = rows * (w + 1) / nt;
+        o.reserve((size_t)(r1 - r0) * nx * 48);
+        for (long r = r0; r < r1; ++r) {
+            const int k = (int)(r / ny), j = (int)(r % ny);
+            yz.clear();
+            append_fixed(yz, (j - (ny - 1) / 2.) * c->latticeSpacingY, 3);
+            yz.push_back('\t');
+            append_fixed(yz, (k - (nz - 1) / 2.) * c->latticeSpacingRapidity, 3);
+            yz.push_back('\t');
+            const double *row = var + (size_t)k * nx * ny + (size_t)j * nx;
             for (int i = 0; i < nx; ++i) {
-                double x = (i - (nx - 1) / 2.) * c->latticeSpacingX;
-                fprintf(fp, "%.3f\t%.3f\t%.3f\t%.8f\n", x, y, z, var[(size_t)k * nx * ny + (size_t)j * nx + i]);
+                o.append(xs[i]);
+                o.append(yz);
+                append_fixed(o, row[i], 8);
+                o.push_back('\n');
             }
         }
-    }
-    fclose(fp);
+    };
+    std::vector<std::thread> pool;
+    for (int w = 1; w < nt; ++w) pool.emplace_back(work, w);
+    work(0);
+    for (auto &th : pool) th.join();
+    int rc = VAH_OK;
+    for (const std::string &o : part)
+        if (fwrite(o.data(), 1, o.size(), fp) != o.size()) rc = VAH_ERR_ARG;
+    if (fclose(fp) != 0) rc = VAH_ERR_ARG;
+    return rc;
 }
 
 void write_bin(const double *var, size_t n, double t, const char *dir, const char *name) {
@@ -149,9 +205,44 @@ void write_bin(const double *var, size_t n, double t, const char *dir, const cha
     fclose(fp);
 }
 
+// The files of one output step, in the order of HydroPlugin.cpp:34-108.
+int write_output(const double *q, const double *e, const double *p, const double *u, size_t n, double t,
+                 const char *dir, int binary, const vah_config *c) {
+    static const char *names[VAH_NQ] = {"ttt",  "ttx",  "tty",  "ttn",  "pl",   "pitt", "pitx", "pity",
+                                        "pitn", "pixx", "pixy", "pixn", "piyy", "piyn", "pinn"};
+    int rc = VAH_OK;
+    auto put = [&](const double *v, const char *name) {
+        if (rc == VAH_OK) rc = write_dat(v, t, dir, name, c);
+    };
+    put(e, "e");
+    put(p, "p");
+    put(u + n, "ux");
+    put(u + 2 * n, "uy");
+    put(u + 3 * n, "un");
+    put(u, "ut");
+    put(q + VAH_PL * n, "pl");
+    const int pi_order[10] = {VAH_PIXX, VAH_PIXY, VAH_PIXN, VAH_PIYY, VAH_PIYN,
+                              VAH_PITT, VAH_PITX, VAH_PITY, VAH_PITN, VAH_PINN};
+    for (int f : pi_order) put(q + (size_t)f * n, names[f]);
+    if (binary && rc == VAH_OK) {
+        write_bin(e, n, t, dir, "e");
+        write_bin(p, n, t, dir, "p");
+        write_bin(u, 4 * n, t, dir, "u");
+        write_bin(q, VAH_NQ * n, t, dir, "q");
+    }
+    return rc;
+}
+
 }  // namespace
 
 extern "C" {
+
+int vah_write_dat(const vah_config *c, const double *var, double tau, const char *dir, const char *name) {
+    if (!c || !var || !dir || !name || c->numLatticePointsX < 1 || c->numLatticePointsY < 1 ||
+        c->numLatticePointsRapidity < 1)
+        return VAH_ERR_ARG;
+    return write_dat(var, tau, dir, name, c);
+}
 
 int vah_config_load(const char *dir, vah_config *c) {
     if (!c) return VAH_ERR_ARG;
@@ -240,50 +331,64 @@ int vah_run(const vah_config *c, const char *outdir, int binary) {
     const int kctr = (nz % 2 == 0) ? ncz / 2 : (ncz - 1) / 2;
     const size_t sctr = (size_t)(kctr - 2) * nx * ny + (size_t)(jctr - 2) * nx + (ictr - 2);
 
+    // Output: the state of an output step is downloaded into (q, ev, pv, u),
+    // then handed to a writer thread that formats and writes its files while
+    // the device runs the next steps; the next output waits for it first.
     std::vector<double> q(VAH_NQ * n), ev(n), pv(n), u(4 * n);
-    static const char *names[VAH_NQ] = {"ttt",  "ttx",  "tty",  "ttn",  "pl",   "pitt", "pitx", "pity",
-                                        "pitn", "pixx", "pixy", "pixn", "piyy", "piyn", "pinn"};
+    std::vector<double> wq(VAH_NQ * n), we(n), wp(n), wu(4 * n);
+    std::thread writer;
+    int wrc = VAH_OK;
+    struct Joiner {
+        std::thread &t;
+        ~Joiner() {
+            if (t.joinable()) t.join();
+        }
+    } joiner{writer};
+    auto finish_writer = [&]() {
+        if (writer.joinable()) writer.join();
+        return wrc;
+    };
+    auto fail = [&](int code) {
+        finish_writer();
+        vah_destroy(ctx);
+        return code;
+    };
     double t = t0, total_ms = 0;
     int nsteps = 0;
     for (int step = 1; step <= nt + 1; ++step) {
         if ((step - 1) % 10 == 0) {
             if ((rc = vah_synchronize(ctx)) != VAH_OK ||
-                (rc = vah_get_state(ctx, q.data(), ev.data(), pv.data(), u.data(), nullptr)) != VAH_OK) {
-                vah_destroy(ctx);
-                return rc;
-            }
+                (rc = vah_get_state(ctx, q.data(), ev.data(), pv.data(), u.data(), nullptr)) != VAH_OK)
+                return fail(rc);
             double Tc = ev[sctr] > 0 ? (double)powf((float)(ev[sctr] / 13.8997), 0.25f) : 0.0;
             printf("n = %d:%d (t = %.3f),\t (e, p) = (%.3f, %.3f) [GeV/fm^3],\t (T = %.3f [GeV]),\t", step - 1, nt, t,
                    ev[sctr] * hbarc, pv[sctr] * hbarc, Tc * hbarc);
-            // HydroPlugin.cpp:34-108 file set and order
-            write_dat(ev.data(), t, outdir, "e", c);
-            write_dat(pv.data(), t, outdir, "p", c);
-            write_dat(u.data() + n, t, outdir, "ux", c);
-            write_dat(u.data() + 2 * n, t, outdir, "uy", c);
-            write_dat(u.data() + 3 * n, t, outdir, "un", c);
-            write_dat(u.data(), t, outdir, "ut", c);
-            write_dat(q.data() + VAH_PL * n, t, outdir, "pl", c);
-            const int pi_order[10] = {VAH_PIXX, VAH_PIXY, VAH_PIXN, VAH_PIYY, VAH_PIYN,
-                                      VAH_PITT, VAH_PITX, VAH_PITY, VAH_PITN, VAH_PINN};
-            for (int f : pi_order) write_dat(q.data() + (size_t)f * n, t, outdir, names[f], c);
-            if (binary) {
-                write_bin(ev.data(), n, t, outdir, "e");
-                write_bin(pv.data(), n, t, outdir, "p");
-                write_bin(u.data(), 4 * n, t, outdir, "u");
-                write_bin(q.data(), VAH_NQ * n, t, outdir, "q");
+            if ((rc = finish_writer()) != VAH_OK) {
+                fprintf(stderr, "vah_run: cannot write the output files in %s\n", outdir);
+                return fail(rc);
             }
+            q.swap(wq);
+            ev.swap(we);
+            pv.swap(wp);
+            u.swap(wu);
+            writer = std::thread([&, t]() {
+                wrc = write_output(wq.data(), we.data(), wp.data(), wu.data(), n, t, outdir, binary, c);
+            });
         }
         auto a = std::chrono::steady_clock::now();
         if ((rc = vah_step(ctx, 1)) != VAH_OK || (rc = vah_synchronize(ctx)) != VAH_OK) {
             fprintf(stderr, "step %d: %s\n", step, vah_last_error());
-            vah_destroy(ctx);
-            return rc;
+            return fail(rc);
         }
         double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - a).count();
         if ((step - 1) % 10 == 0) printf("(Elapsed time/step: %.3f ms)\n", ms);
         total_ms += ms;
         ++nsteps;
         t = t0 + step * dt;
+    }
+    if ((rc = finish_writer()) != VAH_OK) {
+        fprintf(stderr, "vah_run: cannot write the output files in %s\n", outdir);
+        return fail(rc);
     }
     printf("Average time/step: %.3f ms\n", total_ms / (double)nsteps);
     vah_destroy(ctx);

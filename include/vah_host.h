@@ -96,9 +96,17 @@ int vah_config_load(const char *dir, vah_config *cfg);
  * Interior arrays; ut/ux/uy/un may be NULL for types with no flow. */
 int vah_ic_reference(const vah_config *cfg, double *e, double *ut, double *ux, double *uy, double *un);
 
+/* One interior field (x fastest, then y, then eta) as <dir>/<name>_<tau %.3f>.dat
+ * in the reference's text format (output(), rhic/src/FileIO.cpp:14-59): a line
+ * "%.3f\t%.3f\t%.3f\t%.8f\n" of x, y, eta and the value per cell, eta outermost.
+ * Byte for byte printf's output; formatted on up to 16 host threads.
+ * VAH_ERR_ARG for a bad argument or a file that cannot be written. */
+int vah_write_dat(const vah_config *cfg, const double *var, double tau, const char *dir, const char *name);
+
 /* run() of rhic/src/HydroPlugin.cpp:130-242 on the device: initial state from
  * the reference IC, nt+1 steps, every 10 steps the centre-cell printout and
- * the <name>_<tau %.3f>.dat files (FileIO.cpp:14-59) in output_dir.
+ * the <name>_<tau %.3f>.dat files (FileIO.cpp:14-59) in output_dir, written
+ * by a host thread while the device runs the next steps.
  * binary_dumps != 0 also writes full-precision <name>_<tau>.bin files. */
 int vah_run(const vah_config *cfg, const char *output_dir, int binary_dumps);
 

@@ -109,6 +109,50 @@ def test_config_defaults_and_reference_files(tmp_path):
     assert c.initializePimunuNavierStokes == 0
 
 
+def _printf_dat(cfg, var):
+    """The reference's output() lines through the C library's snprintf."""
+    import ctypes
+    import ctypes.util
+    libc = ctypes.CDLL(ctypes.util.find_library("c"))
+    buf = ctypes.create_string_buffer(2048)
+    nx, ny, nz = cfg.numLatticePointsX, cfg.numLatticePointsY, cfg.numLatticePointsRapidity
+    out = []
+    for k in range(nz):
+        z = (k - (nz - 1) / 2.) * cfg.latticeSpacingRapidity
+        for j in range(ny):
+            y = (j - (ny - 1) / 2.) * cfg.latticeSpacingY
+            for i in range(nx):
+                x = (i - (nx - 1) / 2.) * cfg.latticeSpacingX
+                v = var[(k * ny + j) * nx + i]
+                libc.snprintf(buf, 2048, b"%.3f\t%.3f\t%.3f\t%.8f\n", ctypes.c_double(x), ctypes.c_double(y),
+                              ctypes.c_double(z), ctypes.c_double(v))
+                out.append(buf.value)
+    return b"".join(out)
+
+
+def test_dat_writer_is_printf_to_the_byte(tmp_path):
+    """vah_write_dat (the .dat files of vah_run) formats on host threads with
+    std::to_chars; its file equals the C library's printf of the reference's
+    format line by line: random magnitudes over 40 decades, exact decimal
+    ties of the last digit, signed zeros, values too long for the fast path,
+    nan and inf."""
+    V = vah()
+    c = V.config_load(str(tmp_path / "missing"))
+    c.numLatticePointsX, c.numLatticePointsY, c.numLatticePointsRapidity = 37, 23, 9
+    c.latticeSpacingX, c.latticeSpacingY, c.latticeSpacingRapidity = 0.1, 0.13, 0.3
+    n = 37 * 23 * 9
+    rng = np.random.default_rng(4)
+    var = rng.standard_normal(n) * 10.0 ** rng.uniform(-20, 20, n)
+    special = [0.0, -0.0, 5e-9, -5e-9, 0.125e-7, 1.5e-8, 2.5e-8, 0.5, 1e300, -3e250, 1e-300, float("nan"),
+               float("inf"), -float("inf"), 123456789.123456785, 2.0 ** -27, 2.0 ** -30]
+    var[:len(special)] = special
+    var[100:200] = np.round(var[100:200] * 1e8) / 1e8 + 5e-9  # near-ties of the 8th decimal
+    V.write_dat(c, var, 1.25, str(tmp_path), "e")
+    got = (tmp_path / "e_1.250.dat").read_bytes()
+    want = _printf_dat(c, var)
+    assert got == want
+
+
 def test_config_int_literal_for_float_keeps_default(tmp_path):
     """libconfig does not convert an integer literal for a float lookup, so
     the reference keeps the default (Properties.cpp:17-22)."""

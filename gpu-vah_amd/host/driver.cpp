@@ -186,7 +186,12 @@ int write_dat(const double *var, double t, const char *dir, const char *name, co
         }
     };
     std::vector<std::thread> pool;
-    for (int w = 1; w < nt; ++w) pool.emplace_back(work, w);
+    int w = 1;
+    try {
+        for (; w < nt; ++w) pool.emplace_back(work, w);
+    } catch (...) {  // no thread to be had: this one formats the rest
+        for (int v = w; v < nt; ++v) work(v);
+    }
     work(0);
     for (auto &th : pool) th.join();
     int rc = VAH_OK;
@@ -371,9 +376,14 @@ int vah_run(const vah_config *c, const char *outdir, int binary) {
             ev.swap(we);
             pv.swap(wp);
             u.swap(wu);
-            writer = std::thread([&, t]() {
+            auto job = [&, t]() {
                 wrc = write_output(wq.data(), we.data(), wp.data(), wu.data(), n, t, outdir, binary, c);
-            });
+            };
+            try {
+                writer = std::thread(job);
+            } catch (...) {  // no thread to be had: write before stepping on
+                job();
+            }
         }
         auto a = std::chrono::steady_clock::now();
         if ((rc = vah_step(ctx, 1)) != VAH_OK || (rc = vah_synchronize(ctx)) != VAH_OK) {

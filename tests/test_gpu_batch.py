@@ -103,6 +103,34 @@ def test_batch_matches_single_lattice_path_and_oracle(oracle_mod):
     B.close()
 
 
+def test_batch_with_the_reference_eos_fits():
+    """The Wuppertal-Budapest fits of the reference's non-conformal EOS
+    (EquationOfState.cpp:31-142) in the batch engine: a 2+1D and a 3+1D event
+    equal their single-lattice runs with the same EOS to 1e-12 per cell."""
+    V = vah()
+    evs = [dict(nx=30, ny=26, nz=1, dx=0.1, dy=0.1, dz=0.1, dt=0.02, t0=0.5, etabar=0.16,
+                e=gaussian_ic(30, 26, seed=21, peak=60.0, sx=1.2, sy=1.5)),
+           dict(nx=18, ny=16, nz=7, dx=0.1, dy=0.1, dz=0.3, dt=0.02, t0=0.5, etabar=0.16,
+                e=gaussian_ic(18, 16, 7, dz=0.3, seed=22, peak=60.0, sx=0.8, sy=0.9))]
+    steps = 15
+    B = V.Batch(budget_bytes=1 << 30, max_events=4, freezeout_T_GeV=0.0, steps_per_graph=4, eos_kind=V.EOS_WB)
+    slots = [admit(B, ev, max_steps=steps, tag=i) for i, ev in enumerate(evs)]
+    done = run_until_stopped(B, slots, chunk=5)
+    for ev, s in zip(evs, slots):
+        g = V.Vah(ev["nx"], ev["ny"], ev["nz"], ev["dx"], ev["dy"], ev["dz"], ev["dt"], ev["etabar"], ev["t0"],
+                  eos_kind=V.EOS_WB)
+        n = ev["nx"] * ev["ny"] * ev["nz"]
+        g.set_primitives(ev["e"], *rest_frame(n), pl=ev["e"] / 3)
+        g.step(steps)
+        g.synchronize()
+        b = g.state()
+        g.close()
+        err = worst(B.state(s), b, done[s]["tau"])
+        print(f"WB EOS event {ev['nx']}x{ev['ny']}x{ev['nz']}: batch vs single {err:.2e}")
+        assert err <= 1e-12
+    B.close()
+
+
 def test_degenerate_events_in_a_batch(oracle_mod):
     """Events one or two cells wide, down to a single cell, resident beside a
     normal event: each within 1e-10 of the oracle run of the same event (the

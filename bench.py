@@ -623,6 +623,13 @@ def main():
                      "recovery_fallbacks": st.recovery_fallbacks, "nan_cells": st.nan_cells,
                      "max_vT_end": st.max_vT}
     st = g.stats()
+    # the validity maps (checkValidity, which the reference's GPU step runs after
+    # every step) as the on-demand device pass, without host copies
+    g.validity(copy=False)
+    t0v = time.perf_counter()
+    for _ in range(5):
+        g.validity(copy=False)
+    validity_ms = (time.perf_counter() - t0v) / 5 * 1e3
     g.close()
 
     value = n / (ms * 1e-3)
@@ -650,7 +657,8 @@ def main():
                    "reps": args.reps, "rep_ms_per_step": [round(x, 4) for x in rep_ms],
                    "stage_kernel_ms": prof["avg_ms"], "stage_kernel_launches": prof["launches"],
                    "recovery_fallbacks": st.recovery_fallbacks, "nan_cells": st.nan_cells,
-                   "evolution": evolution},
+                   "evolution": evolution,
+                   "validity_pass_ms": round(validity_ms, 4)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None if traffic is None else traffic[f"stage{dom + 1}"],

@@ -393,9 +393,13 @@ class Vah:
         _check(lib().vah_get_state(self.h, *[x.ctypes.data_as(_dp) for x in (q, e, p, u, up)]))
         return {"q": q.reshape(NQ, n), "e": e, "p": p, "u": u.reshape(4, n), "up": up.reshape(4, n)}
 
-    def validity(self):
-        """Knudsen numbers (theta_T, zDzu, theta), R_pi and R_pi2 after the last step."""
+    def validity(self, copy=True):
+        """Knudsen numbers (theta_T, zDzu, theta), R_pi and R_pi2 after the last step.
+        copy=False runs the device pass alone (no host copies) and returns None."""
         keys = ("kn_T", "kn_L", "kn", "r_pi", "r_pi2")
+        if not copy:
+            _check(lib().vah_validity(self.h, *[None] * len(keys)))
+            return None
         out = {k: np.empty(self.n) for k in keys}
         _check(lib().vah_validity(self.h, *[out[k].ctypes.data_as(_dp) for k in keys]))
         return out

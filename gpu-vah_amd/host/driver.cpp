@@ -246,7 +246,11 @@ int vah_write_dat(const vah_config *c, const double *var, double tau, const char
     if (!c || !var || !dir || !name || c->numLatticePointsX < 1 || c->numLatticePointsY < 1 ||
         c->numLatticePointsRapidity < 1)
         return VAH_ERR_ARG;
-    return write_dat(var, tau, dir, name, c);
+    try {
+        return write_dat(var, tau, dir, name, c);
+    } catch (...) {  // out of host memory for the text buffers
+        return VAH_ERR_NOMEM;
+    }
 }
 
 int vah_config_load(const char *dir, vah_config *c) {
@@ -377,7 +381,11 @@ int vah_run(const vah_config *c, const char *outdir, int binary) {
             pv.swap(wp);
             u.swap(wu);
             auto job = [&, t]() {
-                wrc = write_output(wq.data(), we.data(), wp.data(), wu.data(), n, t, outdir, binary, c);
+                try {
+                    wrc = write_output(wq.data(), we.data(), wp.data(), wu.data(), n, t, outdir, binary, c);
+                } catch (...) {  // out of host memory for the text buffers
+                    wrc = VAH_ERR_NOMEM;
+                }
             };
             try {
                 writer = std::thread(job);

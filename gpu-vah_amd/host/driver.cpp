@@ -152,10 +152,6 @@ inline void append_fixed(std::string &out, double v, int prec) {
 // values are formatted in parallel, row blocks per thread, and the blocks
 // written in order.
 int write_dat(const double *var, double t, const char *dir, const char *name, const vah_config *c) {
-    char fname[1024];
-    snprintf(fname, sizeof fname, "%s/%s_%.3f.dat", dir, name, t);
-    FILE *fp = fopen(fname, "w");
-    if (!fp) return VAH_ERR_ARG;
     const int nx = c->numLatticePointsX, ny = c->numLatticePointsY, nz = c->numLatticePointsRapidity;
     std::vector<std::string> xs(nx);
     for (int i = 0; i < nx; ++i) {
@@ -165,7 +161,8 @@ int write_dat(const double *var, double t, const char *dir, const char *name, co
     const long rows = (long)ny * nz;
     const int nt = (int)std::min<long>(writer_threads(), rows);
     std::vector<std::string> part(nt);
-    auto work = [&](int w) {
+    std::vector<char> failed(nt, 0);  // a formatting thread that ran out of memory
+    auto format = [&](int w) {
         std::string &o = part[w], yz;
         const long r0 = rows * w / nt, r1 = rows * (w + 1) / nt;
         o.reserve((size_t)(r1 - r0) * nx * 48);
@@ -185,6 +182,13 @@ int write_dat(const double *var, double t, const char *dir, const char *name, co
             }
         }
     };
+    auto work = [&](int w) {
+        try {
+            format(w);
+        } catch (...) {
+            failed[w] = 1;
+        }
+    };
     std::vector<std::thread> pool;
     int w = 1;
     try {
@@ -194,6 +198,11 @@ int write_dat(const double *var, double t, const char *dir, const char *name, co
     }
     work(0);
     for (auto &th : pool) th.join();
+    if (std::find(failed.begin(), failed.end(), 1) != failed.end()) return VAH_ERR_NOMEM;
+    char fname[1024];
+    snprintf(fname, sizeof fname, "%s/%s_%.3f.dat", dir, name, t);
+    FILE *fp = fopen(fname, "w");
+    if (!fp) return VAH_ERR_ARG;
     int rc = VAH_OK;
     for (const std::string &o : part)
         if (fwrite(o.data(), 1, o.size(), fp) != o.size()) rc = VAH_ERR_ARG;

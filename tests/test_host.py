@@ -153,6 +153,25 @@ def test_dat_writer_is_printf_to_the_byte(tmp_path):
     assert got == want
 
 
+def test_dat_writer_argument_checks(tmp_path):
+    """Bad arguments and an unwritable directory come back as VAH_ERR_ARG."""
+    import ctypes
+    V = vah()
+    L = V.lib()
+    c = V.config_load(str(tmp_path / "missing"))
+    c.numLatticePointsX, c.numLatticePointsY, c.numLatticePointsRapidity = 3, 2, 1
+    v = np.zeros(6)
+    p = v.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    d, nm = str(tmp_path).encode(), b"e"
+    assert L.vah_write_dat(None, p, 0.5, d, nm) == V.VAH_ERR_ARG
+    assert L.vah_write_dat(ctypes.byref(c), None, 0.5, d, nm) == V.VAH_ERR_ARG
+    assert L.vah_write_dat(ctypes.byref(c), p, 0.5, None, nm) == V.VAH_ERR_ARG
+    assert L.vah_write_dat(ctypes.byref(c), p, 0.5, str(tmp_path / "no" / "such").encode(), nm) == V.VAH_ERR_ARG
+    c.numLatticePointsY = 0
+    assert L.vah_write_dat(ctypes.byref(c), p, 0.5, d, nm) == V.VAH_ERR_ARG
+    assert not list(tmp_path.glob("*.dat"))
+
+
 def test_config_int_literal_for_float_keeps_default(tmp_path):
     """libconfig does not convert an integer literal for a float lookup, so
     the reference keeps the default (Properties.cpp:17-22)."""

@@ -134,10 +134,10 @@ __device__ __forceinline__ int newton_update(double t, const FaceState &S, doubl
     double f = U * g - xM;
     double fp = iP * g - 0.5 * U * b * xM * iP;
     fp = fabs(fp) == 0.0 ? 1.e-16 : fp;
-    double u = u0 - fdiv(f, fp);
+    double u = u0 - newton_quotient(f, fp);
     *u1 = u;
     // non-short-circuit operators: no branch splits the two sides' chains
-    const bool bad = (e < 0.0) | __builtin_isnan(u) | __builtin_isinf(u) | (u < 0.0) | (u > 9.0072e+15);
+    const bool bad = (e < 0.0) | newton_out_of_range(u);
     const bool conv = fabs(u - u0) <= 1.e-7 * fabs(u);
     return bad ? -1 : (int)conv;
 }

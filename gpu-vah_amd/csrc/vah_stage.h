@@ -142,16 +142,41 @@ __device__ __forceinline__ int newton_update(double t, const FaceState &S, doubl
     return bad ? -1 : (int)conv;
 }
 
+// A start for the face solve at its root.  With Pi = 0 the function the
+// update zeroes is a quadratic in U = u / sqrt(1 + u^2):
+//     f = U (bL + b (e - PL) / 2) - xM,  e = c0 - U xM
+//       = -(b xM / 2) U^2 + B U - xM,    B = bL + b (c0 - PL) / 2,
+// whose smaller root 2 xM / (B + sqrt(B^2 - 2 b xM^2)) is the physical one
+// (the roots' product is 2 / b = 2 for Ft^2 <= 1, and U < 1 for every finite
+// u).  Newton started there stops after one or two passes instead of three
+// or four from a neighbour cell's |u_T|, and ends on the same root; where
+// the quadratic has no root in [0, 1), `other` is the start as before.
+__device__ __forceinline__ double root_start(double t, const FaceState &S, double other) {
+    const double Ft2 = S.Ft * S.Ft;
+    const double xM = S.x * S.MBT;
+    const double bL = S.x * S.x * S.M0 - Ft2 * S.PL;
+    const double b = S.x * S.x + Ft2;
+    const double c0 = S.M0 - t * S.Ft * S.M3;
+    const double B = bL + 0.5 * b * (c0 - S.PL);
+    const double disc = B * B - 2.0 * b * xM * xM;
+    const double U = fdiv(2.0 * xM, B + sqrt_pos(fmax(disc, 0.0)));
+    const bool ok = (disc >= 0.0) & (B > 0.0) & (U >= 0.0) & (U < 1.0);
+    const double u = U * rsqrt(fmax(1.0 - U * U, 1.e-300));
+    return ok ? u : other;
+}
+
 // Both sides of a face solved in one loop (independent iterations, each side
-// stops at its own convergence or failure, at most 10^4 iterations each).
+// stops at its own convergence or failure, at most 10^4 iterations each),
+// from `guess` or, with ROOT_START, from each side's root_start.
 // Every pass evaluates both updates and keeps each only while its side is
 // still running, so the two dependency chains interleave and the loop body
 // has no branches.  Outputs the signed speed v = u^DIR / u^tau of each side;
 // a failed recovery gives the reference's fallback u = (1,0,0,0), i.e. v = 0.
-template <int DIR>
+template <int DIR, bool ROOT_START>
 __device__ __forceinline__ void face_speeds(double t, double guess, const FaceState &A, const FaceState &B, double &vA,
                                             double &vB) {
-    double ua = guess, ub = guess;
+    double ua = ROOT_START ? root_start(t, A, guess) : guess;
+    double ub = ROOT_START ? root_start(t, B, guess) : guess;
     int sa = 0, sb = 0;
     for (int it = 0; it < 10000 && (sa == 0 || sb == 0); ++it) {
         double na, nb;
@@ -233,7 +258,7 @@ __device__ __forceinline__ Views make_views(const StageParams &P) {
 // (HalfSiteExtrapolation.cpp:12-26); emit(n, H_n) receives the 15 components
 // in order.  The nine components the Newton solve needs stay in registers,
 // the other six are reconstructed after it, which keeps the live state small.
-template <int DIR, typename R, typename Guess, typename Load, typename Emit>
+template <int DIR, typename R, bool ROOT_START, typename Guess, typename Load, typename Emit>
 __device__ __forceinline__ void face_flux_from(double t, Guess guess, Load load, Emit emit) {
     constexpr int NN = 9;  // TTT..PITN
     // all loads of the nine fields first, so that they are in flight together;
@@ -257,7 +282,7 @@ __device__ __forceinline__ void face_flux_from(double t, Guess guess, Load load,
     FaceState FR = face_state(t, qR[TTT], qR[TTX], qR[TTY], qR[TTN], qR[PL], qR[PITT], qR[PITX], qR[PITY], qR[PITN]);
     FaceState FL = face_state(t, qL[TTT], qL[TTX], qL[TTY], qL[TTN], qL[PL], qL[PITT], qL[PITX], qL[PITY], qL[PITN]);
     double vR, vL;
-    face_speeds<DIR>(t, uT_guess, FR, FL, vR, vL);
+    face_speeds<DIR, ROOT_START>(t, uT_guess, FR, FL, vR, vL);
     const double a = fmax(fabs(vL), fabs(vR));
     // (F(qR) + F(qL) - a (qR - qL)) / 2 with F(q) = v q
     const R cR = (R)(0.5 * (vR - a)), cL = (R)(0.5 * (vL + a));
@@ -283,10 +308,10 @@ __device__ __forceinline__ void face_flux_from(double t, Guess guess, Load load,
 // (HalfSiteExtrapolation.cpp:12-26); emit(n, H_n) receives the 15 components
 // in order.  The nine components the Newton solve needs stay in registers,
 // the other six are loaded and reconstructed after it.
-template <int DIR, typename R, typename Guess, typename Emit>
+template <int DIR, typename R, bool ROOT_START, typename Guess, typename Emit>
 __device__ __forceinline__ void face_flux(double t, Guess guess, const Buf &q, uint32_t f8, uint32_t c,
                                           uint32_t stp, Emit emit) {
-    face_flux_from<DIR, R>(
+    face_flux_from<DIR, R, ROOT_START>(
         t, guess, [&](int n, int j) { return ld(q, c + (uint32_t)j * stp - 2u * stp, n * f8); }, emit);
 }
 
@@ -583,7 +608,7 @@ __device__ __forceinline__ void stage_tile(const StageParams &P, int tile, R (*H
                 const int gi = i0 + fi, gj = j0 + fj;  // interior index of the right cell
                 if (fj < TY && gi <= P.nx && gj < P.ny) {
                     const uint32_t c = (plane + (uint32_t)(gj + 2) * sy + (uint32_t)(gi + 2)) * 8u;
-                    face_flux<1, R>(t, [&] { return uT_of(V.ucur, f8, c - 8); }, V.qin, f8, c, 8,
+                    face_flux<1, R, true>(t, [&] { return uT_of(V.ucur, f8, c - 8); }, V.qin, f8, c, 8,
                                     [&](int n, R h) { Hs[n][tid] = h; });
                 }
             }
@@ -600,7 +625,7 @@ __device__ __forceinline__ void stage_tile(const StageParams &P, int tile, R (*H
                 const int gi = i0 + fi, gj = j0 + fj;
                 if (fj <= TY && gi < P.nx && gj <= P.ny) {
                     const uint32_t c = (plane + (uint32_t)(gj + 2) * sy + (uint32_t)(gi + 2)) * 8u;
-                    face_flux<2, R>(t, [&] { return uT_of(V.ucur, f8, c - sy8); }, V.qin, f8, c, sy8,
+                    face_flux<2, R, true>(t, [&] { return uT_of(V.ucur, f8, c - sy8); }, V.qin, f8, c, sy8,
                                     [&](int n, R h) { Hs[n][tid] = h; });
                 }
             }
@@ -625,13 +650,13 @@ __device__ __forceinline__ void stage_tile(const StageParams &P, int tile, R (*H
 #pragma unroll
                             for (int n = 0; n < NQ; ++n) zr(kk + j - 2, n) = (R)v[n];
                         }
-                        face_flux_from<3, R>(t, [&] { return uT_of(V.ucur, f8, s - sz8); },
+                        face_flux_from<3, R, true>(t, [&] { return uT_of(V.ucur, f8, s - sz8); },
                             [&](int n, int j) {
                                 return j == 0 ? ld(V.qin, s - 2u * sz8, n * f8) : (double)zr(kk + j - 2, n);
                             },
                             [&](int n, R h) { Hz[n][tid] = h; });
                     } else {
-                        face_flux<3, R>(t, [&] { return uT_of(V.ucur, f8, s - sz8); }, V.qin, f8, s, sz8,
+                        face_flux<3, R, true>(t, [&] { return uT_of(V.ucur, f8, s - sz8); }, V.qin, f8, s, sz8,
                                         [&](int n, R h) { Hz[n][tid] = h; });
                     }
                 }
@@ -645,12 +670,12 @@ __device__ __forceinline__ void stage_tile(const StageParams &P, int tile, R (*H
                     double fresh[NQ];
 #pragma unroll
                     for (int n = 0; n < NQ; ++n) fresh[n] = ld(V.qin, s + 2u * sz8, n * f8);
-                    face_flux_from<3, R>(t, [&] { return uT_of(V.ucur, f8, s); },
+                    face_flux_from<3, R, true>(t, [&] { return uT_of(V.ucur, f8, s); },
                         [&](int n, int j) { return j < 3 ? (double)zr(kk - 1 + j, n) : fresh[n]; }, upper);
 #pragma unroll
                     for (int n = 0; n < NQ; ++n) zr(kk + 2, n) = (R)fresh[n];
                 } else {
-                    face_flux<3, R>(t, [&] { return uT_of(V.ucur, f8, s); }, V.qin, f8, s + sz8, sz8, upper);
+                    face_flux<3, R, true>(t, [&] { return uT_of(V.ucur, f8, s); }, V.qin, f8, s + sz8, sz8, upper);
                 }
             }
         } else if (cell_ok) {
@@ -659,22 +684,22 @@ __device__ __forceinline__ void stage_tile(const StageParams &P, int tile, R (*H
             {
                 R Sa[4];
                 axis_source<1>(axis_pi<1, R>(V.qin, f8, s, 8, P.hx), (R)(ld(V.ucur, s, f8) * iut), Sa);
-                face_flux<1, R>(t, [&] { return uTs; }, V.qin, f8, s + 8, 8, [&](int n, R h) { Hs[n][tid] = h; });
-                face_flux<1, R>(t, [&] { return uTs; }, V.qin, f8, s, 8,
+                face_flux<1, R, false>(t, [&] { return uTs; }, V.qin, f8, s + 8, 8, [&](int n, R h) { Hs[n][tid] = h; });
+                face_flux<1, R, false>(t, [&] { return uTs; }, V.qin, f8, s, 8,
                                 [&](int n, R h) { add_component(acc[n], n, h, Hs[n][tid], Sa, inv_dx, dt); });
             }
             {
                 R Sa[4];
                 axis_source<2>(axis_pi<2, R>(V.qin, f8, s, sy8, P.hy), (R)(ld(V.ucur, s, 2 * f8) * iut), Sa);
-                face_flux<2, R>(t, [&] { return uTs; }, V.qin, f8, s + sy8, sy8, [&](int n, R h) { Hs[n][tid] = h; });
-                face_flux<2, R>(t, [&] { return uTs; }, V.qin, f8, s, sy8,
+                face_flux<2, R, false>(t, [&] { return uTs; }, V.qin, f8, s + sy8, sy8, [&](int n, R h) { Hs[n][tid] = h; });
+                face_flux<2, R, false>(t, [&] { return uTs; }, V.qin, f8, s, sy8,
                                 [&](int n, R h) { add_component(acc[n], n, h, Hs[n][tid], Sa, inv_dy, dt); });
             }
             if (D3) {
                 R Sa[4];
                 axis_source<3>(axis_pi<3, R>(V.qin, f8, s, sz8, P.hz), (R)(ld(V.ucur, s, 3 * f8) * iut), Sa);
-                face_flux<3, R>(t, [&] { return uTs; }, V.qin, f8, s + sz8, sz8, [&](int n, R h) { Hs[n][tid] = h; });
-                face_flux<3, R>(t, [&] { return uTs; }, V.qin, f8, s, sz8,
+                face_flux<3, R, false>(t, [&] { return uTs; }, V.qin, f8, s + sz8, sz8, [&](int n, R h) { Hs[n][tid] = h; });
+                face_flux<3, R, false>(t, [&] { return uTs; }, V.qin, f8, s, sz8,
                                 [&](int n, R h) { add_component(acc[n], n, h, Hs[n][tid], Sa, inv_dz, dt); });
             }
         }

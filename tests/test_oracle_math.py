@@ -78,6 +78,46 @@ def test_newton_rest_frame_is_exact(oracle_mod):
         assert out[0] == e and out[2] == 1.0 and out[3] == 0.0 and out[5] == 0.0
 
 
+def face_root_closed_form(t, q):
+    """|u_T| of a state from the quadratic the face solve's update zeroes
+    (Pi = 0): with U = u/sqrt(1+u^2) and M = T^{tau mu} - pi^{tau mu},
+    f(U) = -(b xM/2) U^2 + B U - xM, B = bL + b (c0 - P_L)/2; its smaller
+    root 2 xM / (B + sqrt(B^2 - 2 b xM^2)).  The stage kernels start each
+    face's Newton solve there (gpu-vah_amd/csrc/vah_stage.h, root_start)."""
+    M0, M1, M2, M3 = q[0] - q[5], q[1] - q[6], q[2] - q[7], q[3] - q[8]
+    PL = q[4]
+    Ft = t * M3 / (M0 + PL)
+    x = np.sqrt(abs(1 - Ft * Ft))
+    xM = x * np.hypot(M1, M2)
+    bL = x * x * M0 - Ft * Ft * PL
+    b = x * x + Ft * Ft
+    c0 = M0 - t * Ft * M3
+    B = bL + 0.5 * b * (c0 - PL)
+    U = 2 * xM / (B + np.sqrt(B * B - 2 * b * xM * xM))
+    return U / np.sqrt(1 - U * U)
+
+
+def test_face_solve_root_in_closed_form(oracle_mod):
+    """The closed-form root is the state's |u_T| (the primitives it was built
+    from) and the oracle's Newton solution (EnergyMomentumTensor.cpp:30-72)
+    from a guess 20 % off; a larger root (the other sign) would miss both."""
+    L = oracle_mod.lib()
+    rng = np.random.default_rng(33)
+    out = np.zeros(6)
+    worst_true = worst_newton = 0.0
+    for _ in range(3000):
+        t = rng.uniform(0.3, 5.0)
+        e, pl, ut, ux, uy, un, pi = random_primitives(rng, t)
+        q = conserved_from_primitives(t, e, pl, ut, ux, uy, un, pi)
+        uT = np.hypot(ux, uy)
+        u = face_root_closed_form(t, q)
+        assert L.orc_recover(t, p(q), e, uT * rng.uniform(0.8, 1.2), p(out)) == 0
+        worst_true = max(worst_true, abs(u - uT) / max(uT, 1e-3))
+        worst_newton = max(worst_newton, abs(u - np.hypot(out[3], out[4])) / max(uT, 1e-3))
+    assert worst_true < 1e-12, worst_true
+    assert worst_newton < 1e-12, worst_newton
+
+
 def test_newton_fallback(oracle_mod):
     """Negative energy: the reference falls back to e = 0.999 e_prev, u = (1,0,0,0)
     (EnergyMomentumTensor.cpp:163-172)."""

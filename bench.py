@@ -277,7 +277,8 @@ def batch_sample(V, n_events, tab, autotune_steps=0, max_steps=1000):
                     "each to max T < 150 MeV, one batch",
         "precision": "fp64", "events": n_events, "frozen": frozen,
         "steps_min_max": [int(min(nsteps)), int(max(nsteps))],
-        "events_per_s": n_events / r64["wall"], "wall_s": r64["wall"], "cell_updates_per_s": cu / r64["wall"],
+        "events_per_s": n_events / r64["wall"], "wall_s": r64["wall"], "admission_s": r64["admit_s"],
+        "cell_updates_per_s": cu / r64["wall"],
         "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                       "note": "all events admitted at once; admission to the poll that saw the event frozen"},
         "autotune": r64["autotune"],

@@ -400,7 +400,9 @@ int staging(vah_batch *b, size_t bytes, vah_batch::Staging **out) {
         if (st.host) (void)hipHostFree(st.host);
         st.host = nullptr;
         st.cap = 0;
-        const size_t cap = bytes + bytes / 4;
+        // at least 8 MB: events of mixed sizes then rarely regrow a buffer
+        // (each regrowth frees and allocates pinned memory, milliseconds apiece)
+        const size_t cap = std::max(bytes + bytes / 4, (size_t)8 << 20);
         if (hipHostMalloc((void **)&st.host, cap, hipHostMallocDefault) != hipSuccess)
             return fail(VAH_ERR_NOMEM, "vah_batch_admit: %.1f MB of pinned staging", cap / 1e6);
         st.cap = cap;

@@ -1,12 +1,16 @@
 // vah_batch_variant.hip -- instantiates the persistent batch stage kernels for
-// one tile shape and occupancy target.  Built once per variant with
+// one tile shape and occupancy target.  Built twice per variant with
 //   -DVAH_BV_TX=<tile x> -DVAH_BV_TY=<tile y> -DVAH_BV_B=<workgroups per CU>
-//   -DVAH_BV_FN=<name of the launch function>
-// (the variant table in vah_batch.hip lists the builds).
+//   -DVAH_BV_FN=<name of the launch function> -DVAH_BV_FN3=<its 3+1D part>
+// and -DVAH_BV_PART=2 (the launch function and the 2+1D kernels) or
+// -DVAH_BV_PART=3 (the 3+1D kernels), so that the two dimensionalities can be
+// compiled with different scheduler settings (the variant table in
+// vah_batch.hip lists the builds).
 #include "vah_batch_kernels.h"
 
-#if !defined(VAH_BV_TX) || !defined(VAH_BV_TY) || !defined(VAH_BV_B) || !defined(VAH_BV_FN)
-#error "vah_batch_variant.hip needs VAH_BV_TX, VAH_BV_TY, VAH_BV_B and VAH_BV_FN"
+#if !defined(VAH_BV_TX) || !defined(VAH_BV_TY) || !defined(VAH_BV_B) || !defined(VAH_BV_FN) || \
+    !defined(VAH_BV_FN3) || !defined(VAH_BV_PART)
+#error "vah_batch_variant.hip needs VAH_BV_TX, VAH_BV_TY, VAH_BV_B, VAH_BV_FN, VAH_BV_FN3 and VAH_BV_PART"
 #endif
 
 namespace vah {
@@ -24,24 +28,34 @@ void launch_one(const BatchLaunch &L) {
                        0, L.stream, L.rec, D3 ? L.tiles3 : L.tiles2, L.counts);
 }
 
-template <typename R>
-void launch_r(const BatchLaunch &L, bool d3, bool stage2) {
-    if (d3) {
-        if (stage2) launch_one<true, true, R>(L);
-        else launch_one<true, false, R>(L);
-    } else {
-        if (stage2) launch_one<false, true, R>(L);
-        else launch_one<false, false, R>(L);
-    }
+template <bool D3, typename R>
+void launch_r(const BatchLaunch &L, bool stage2) {
+    if (stage2)
+        launch_one<D3, true, R>(L);
+    else
+        launch_one<D3, false, R>(L);
 }
 
 }  // namespace
 
-void VAH_BV_FN(const BatchLaunch &L, bool d3, bool stage2, bool fp32) {
+#if VAH_BV_PART == 3
+void VAH_BV_FN3(const BatchLaunch &L, bool stage2, bool fp32) {
     if (fp32)
-        launch_r<float>(L, d3, stage2);
+        launch_r<true, float>(L, stage2);
     else
-        launch_r<double>(L, d3, stage2);
+        launch_r<true, double>(L, stage2);
 }
+#else
+void VAH_BV_FN3(const BatchLaunch &L, bool stage2, bool fp32);
+
+void VAH_BV_FN(const BatchLaunch &L, bool d3, bool stage2, bool fp32) {
+    if (d3)
+        VAH_BV_FN3(L, stage2, fp32);
+    else if (fp32)
+        launch_r<false, float>(L, stage2);
+    else
+        launch_r<false, double>(L, stage2);
+}
+#endif
 
 }  // namespace vah

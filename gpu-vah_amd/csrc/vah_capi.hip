@@ -323,10 +323,10 @@ void launch_variant(const LaunchCfg &L, const StageParams &P) {
         launch<D3, S2, false, double>(L, P);
 }
 
+// 2+1D here; 3+1D in vah_stage3d.hip (compiled with other scheduler settings)
 void launch_stage(const LaunchCfg &L, int stage, const StageParams &P) {
     if (L.d3) {
-        if (stage == 1) launch_variant<true, false>(L, P);
-        else launch_variant<true, true>(L, P);
+        launch_stage_3d(StageLaunch{L.grid, L.stream, L.precision, L.flux_mode}, stage, P);
     } else {
         if (stage == 1) launch_variant<false, false>(L, P);
         else launch_variant<false, true>(L, P);

@@ -764,4 +764,14 @@ static __global__ void ghost_fill_kernel(double *q, double *e, double *p, double
     }
 }
 
+// The launches of the 3+1D stage kernels (vah_stage3d.hip): a translation
+// unit of their own, compiled with the scheduler settings that suit them.
+// precision and flux_mode take the values of include/vah.h.
+struct StageLaunch {
+    int grid;
+    hipStream_t stream;
+    int precision, flux_mode;
+};
+void launch_stage_3d(const StageLaunch &L, int stage, const StageParams &P);
+
 }  // namespace vah

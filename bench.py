@@ -139,6 +139,11 @@ def cpu_baseline(w, e, eos_table, seconds=15.0):
 # ---------------------------------------------------------------------------
 HBARC = 0.197326938
 T_FREEZE_GEV = 0.150
+# A request retires at freeze-out or after this many steps, the cap of the
+# batched workload: an event whose dilute edge turns unstable under the
+# reference scheme (the oracle shows the same breakdown) stays hot and would
+# otherwise run until it produces NaN.
+SERVE_MAX_STEPS = 1000
 
 
 def mc_event(V, L, nz, dx, dz, b, seed, width=1.0):
@@ -325,7 +330,7 @@ def serving_sample(V, n_req, rate, tab, budget_gb):
         "workload": f"{n_req} Poisson requests at {rate:g}/s, 80% boost-invariant L^2 (L U{{121..301}}), "
                     "20% 3+1D L^2 x nz (L U{121..201}, nz U{41..81}), dx=deta=0.1, dtau=0.02, MC-Glauber "
                     "(1 fm nucleons) per-request seeds generated on the device at admission, retire at "
-                    "max T < 150 MeV; C++ request loop (vah_serve)",
+                    f"max T < 150 MeV or after {SERVE_MAX_STEPS} steps; C++ request loop (vah_serve)",
         "precision": "fp64", "requests": n_req, "requests_3d": int(sum(r.nz > 1 for r in reqs)),
         "events_per_s": out["events_per_s"],
         "latency_s": {"p50": out["p50_latency_s"], "p99": out["p99_latency_s"],
@@ -373,14 +378,15 @@ def serving_requests(V, n, seed, device_ic=False):
         specs.append((i, L, nz, float(rng.uniform(0.0, 12.0)), seed * 100003 + i))
     base = rng.exponential(1.0, size=n).cumsum()
     if device_ic:
-        reqs = [serve.Request(i, 0.0, L, L, nz, 0.1, 0.1, 0.02, 0.5, 0.16, None,
+        reqs = [serve.Request(i, 0.0, L, L, nz, 0.1, 0.1, 0.02, 0.5, 0.16, None, max_steps=SERVE_MAX_STEPS,
                               glauber=V.glauber(A=208, R=6.62, a=0.546, sigma_nn_mb=70.0, b=b, seed=sd,
                                                 e_peak=60.0 / HBARC, nucleon_width=1.0, eta_flat=1.5, eta_sigma=1.4,
                                                 e_floor=1e-3))
                 for (i, L, nz, b, sd) in specs]
         return reqs, base
     es = parallel_map(lambda sp: mc_event(V, sp[1], sp[2], 0.1, 0.1, sp[3], sp[4]), specs)
-    reqs = [serve.Request(i, 0.0, L, L, nz, 0.1, 0.1, 0.02, 0.5, 0.16, e) for (i, L, nz, _, _), e in zip(specs, es)]
+    reqs = [serve.Request(i, 0.0, L, L, nz, 0.1, 0.1, 0.02, 0.5, 0.16, e, max_steps=SERVE_MAX_STEPS)
+            for (i, L, nz, _, _), e in zip(specs, es)]
     return reqs, base
 
 
@@ -470,7 +476,7 @@ def bench_serving(V, args):
         "config": {"workload": f"serving: {len(reqs)} Poisson requests, 80% boost-invariant L^2 (L U{{121..301}}), "
                                "20% 3+1D L^2 x nz (L U{121..201}, nz U{41..81}), dx=deta=0.1, dtau=0.02, "
                                "MC-Glauber (1 fm nucleon width) per-request seeds generated on the device at "
-                               f"admission, retire at max T < 150 MeV, {args.precision}, C++ request loop, "
+                               f"admission, retire at max T < 150 MeV or after {SERVE_MAX_STEPS} steps, {args.precision}, C++ request loop, "
                                f"HBM budget {args.budget_gb} GiB",
                    "rates_per_s": rates},
         "latency_s": {"p50": top["p50_latency_s"], "p99": top["p99_latency_s"]},

@@ -142,42 +142,65 @@ __device__ __forceinline__ int newton_update(double t, const FaceState &S, doubl
     return bad ? -1 : (int)conv;
 }
 
-// A start for the face solve at its root.  With Pi = 0 the function the
-// update zeroes is a quadratic in U = u / sqrt(1 + u^2):
+// The root a face solve's Newton path ends on, and when the path is known to
+// end there.  With Pi = 0 the function the update zeroes is a quadratic in
+// U = u / sqrt(1 + u^2):
 //     f = U (bL + b (e - PL) / 2) - xM,  e = c0 - U xM
 //       = -(b xM / 2) U^2 + B U - xM,    B = bL + b (c0 - PL) / 2,
-// whose smaller root 2 xM / (B + sqrt(B^2 - 2 b xM^2)) is the physical one
-// (the roots' product is 2 / b = 2 for Ft^2 <= 1, and U < 1 for every finite
-// u).  Newton started there stops after one or two passes instead of three
-// or four from a neighbour cell's |u_T|, and ends on the same root; where
-// the quadratic has no root in [0, 1), `other` is the start as before.
-__device__ __forceinline__ double root_start(double t, const FaceState &S, double other) {
+// whose smaller root U* = 2 xM / (B + sqrt(B^2 - 2 b xM^2)) is the physical
+// one (the roots' product is 2 / b = 2 for Ft^2 <= 1, and U < 1 for every
+// finite u).  On 0 <= U <= U_v = B / (b xM), left of the vertex, f is
+// increasing and concave in u, so from a start there every Newton iterate
+// after the first lies between the first update and the root: e, which falls
+// as U grows, stays at or above e(U*) on all of them and u stays in [0, u*].
+// A path whose start lies there, whose first update passed the tests and is
+// clearly positive, and whose root has e(U*) clearly positive, can therefore
+// neither fail nor end anywhere but on the root.
+struct FaceRoot {
+    double u;    // u* (when ok)
+    double B;    // the vertex test: start U0 with U0 * bxM <= B
+    double bxM;
+    bool ok;     // a root in [0, 1) with e(U*) clearly positive
+};
+
+__device__ __forceinline__ FaceRoot face_root(double t, const FaceState &S) {
     const double Ft2 = S.Ft * S.Ft;
     const double xM = S.x * S.MBT;
     const double bL = S.x * S.x * S.M0 - Ft2 * S.PL;
     const double b = S.x * S.x + Ft2;
     const double c0 = S.M0 - t * S.Ft * S.M3;
-    const double B = bL + 0.5 * b * (c0 - S.PL);
-    const double disc = B * B - 2.0 * b * xM * xM;
-    const double U = fdiv(2.0 * xM, B + sqrt_pos(fmax(disc, 0.0)));
-    const bool ok = (disc >= 0.0) & (B > 0.0) & (U >= 0.0) & (U < 1.0);
-    const double u = U * rsqrt(fmax(1.0 - U * U, 1.e-300));
-    return ok ? u : other;
+    FaceRoot r;
+    r.B = bL + 0.5 * b * (c0 - S.PL);
+    r.bxM = b * xM;
+    const double disc = r.B * r.B - 2.0 * b * xM * xM;
+    const double U = fdiv(2.0 * xM, r.B + sqrt_pos(fmax(disc, 0.0)));
+    r.ok = (disc >= 0.0) & (r.B > 0.0) & (U >= 0.0) & (U < 1.0) & (c0 - U * xM > 1.e-12 * fabs(c0));
+    r.u = U * rsqrt(fmax(1.0 - U * U, 1.e-300));
+    return r;
 }
 
 // Both sides of a face solved in one loop (independent iterations, each side
 // stops at its own convergence or failure, at most 10^4 iterations each),
-// from `guess` or, with ROOT_START, from each side's root_start.
-// Every pass evaluates both updates and keeps each only while its side is
-// still running, so the two dependency chains interleave and the loop body
-// has no branches.  Outputs the signed speed v = u^DIR / u^tau of each side;
-// a failed recovery gives the reference's fallback u = (1,0,0,0), i.e. v = 0.
-template <int DIR, bool ROOT_START>
+// from `guess`.  Every pass evaluates both updates and keeps each only while
+// its side is still running, so the two dependency chains interleave and the
+// loop body has no branches.  With ROOT_JUMP a side whose first pass shows
+// that its path must end on the root (face_root) takes the root there and
+// stops: two to three passes fewer, and every decision (which sides fail)
+// is the one the full path makes.  Outputs the signed speed v = u^DIR / u^tau
+// of each side; a failed recovery gives the reference's fallback
+// u = (1,0,0,0), i.e. v = 0.
+template <int DIR, bool ROOT_JUMP>
 __device__ __forceinline__ void face_speeds(double t, double guess, const FaceState &A, const FaceState &B, double &vA,
                                             double &vB) {
-    double ua = ROOT_START ? root_start(t, A, guess) : guess;
-    double ub = ROOT_START ? root_start(t, B, guess) : guess;
+    double ua = guess, ub = guess;
     int sa = 0, sb = 0;
+    FaceRoot qa, qb;
+    double U0 = 0.0;
+    if (ROOT_JUMP) {
+        qa = face_root(t, A);
+        qb = face_root(t, B);
+        U0 = guess * rsqrt(1.0 + guess * guess);
+    }
     for (int it = 0; it < 10000 && (sa == 0 || sb == 0); ++it) {
         double na, nb;
         const int ta = newton_update(t, A, ua, &na);
@@ -187,6 +210,14 @@ __device__ __forceinline__ void face_speeds(double t, double guess, const FaceSt
         sa = ra ? ta : sa;
         ub = rb ? nb : ub;
         sb = rb ? tb : sb;
+        if (ROOT_JUMP) {
+            const bool ja = (it == 0) & (sa == 0) & qa.ok & (U0 * qa.bxM <= qa.B) & (ua >= 1.e-9 * guess);
+            const bool jb = (it == 0) & (sb == 0) & qb.ok & (U0 * qb.bxM <= qb.B) & (ub >= 1.e-9 * guess);
+            ua = ja ? qa.u : ua;
+            sa = ja ? 1 : sa;
+            ub = jb ? qb.u : ub;
+            sb = jb ? 1 : sb;
+        }
     }
     auto speed = [](const FaceState &S, double u, int st) {
         if (st != 1) return 0.0;
@@ -258,7 +289,7 @@ __device__ __forceinline__ Views make_views(const StageParams &P) {
 // (HalfSiteExtrapolation.cpp:12-26); emit(n, H_n) receives the 15 components
 // in order.  The nine components the Newton solve needs stay in registers,
 // the other six are reconstructed after it, which keeps the live state small.
-template <int DIR, typename R, bool ROOT_START, typename Guess, typename Load, typename Emit>
+template <int DIR, typename R, bool ROOT_JUMP, typename Guess, typename Load, typename Emit>
 __device__ __forceinline__ void face_flux_from(double t, Guess guess, Load load, Emit emit) {
     constexpr int NN = 9;  // TTT..PITN
     // all loads of the nine fields first, so that they are in flight together;
@@ -282,7 +313,7 @@ __device__ __forceinline__ void face_flux_from(double t, Guess guess, Load load,
     FaceState FR = face_state(t, qR[TTT], qR[TTX], qR[TTY], qR[TTN], qR[PL], qR[PITT], qR[PITX], qR[PITY], qR[PITN]);
     FaceState FL = face_state(t, qL[TTT], qL[TTX], qL[TTY], qL[TTN], qL[PL], qL[PITT], qL[PITX], qL[PITY], qL[PITN]);
     double vR, vL;
-    face_speeds<DIR, ROOT_START>(t, uT_guess, FR, FL, vR, vL);
+    face_speeds<DIR, ROOT_JUMP>(t, uT_guess, FR, FL, vR, vL);
     const double a = fmax(fabs(vL), fabs(vR));
     // (F(qR) + F(qL) - a (qR - qL)) / 2 with F(q) = v q
     const R cR = (R)(0.5 * (vR - a)), cL = (R)(0.5 * (vL + a));
@@ -308,10 +339,10 @@ __device__ __forceinline__ void face_flux_from(double t, Guess guess, Load load,
 // (HalfSiteExtrapolation.cpp:12-26); emit(n, H_n) receives the 15 components
 // in order.  The nine components the Newton solve needs stay in registers,
 // the other six are loaded and reconstructed after it.
-template <int DIR, typename R, bool ROOT_START, typename Guess, typename Emit>
+template <int DIR, typename R, bool ROOT_JUMP, typename Guess, typename Emit>
 __device__ __forceinline__ void face_flux(double t, Guess guess, const Buf &q, uint32_t f8, uint32_t c,
                                           uint32_t stp, Emit emit) {
-    face_flux_from<DIR, R, ROOT_START>(
+    face_flux_from<DIR, R, ROOT_JUMP>(
         t, guess, [&](int n, int j) { return ld(q, c + (uint32_t)j * stp - 2u * stp, n * f8); }, emit);
 }
 

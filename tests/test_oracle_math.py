@@ -82,8 +82,9 @@ def face_root_closed_form(t, q):
     """|u_T| of a state from the quadratic the face solve's update zeroes
     (Pi = 0): with U = u/sqrt(1+u^2) and M = T^{tau mu} - pi^{tau mu},
     f(U) = -(b xM/2) U^2 + B U - xM, B = bL + b (c0 - P_L)/2; its smaller
-    root 2 xM / (B + sqrt(B^2 - 2 b xM^2)).  The stage kernels start each
-    face's Newton solve there (gpu-vah_amd/csrc/vah_stage.h, root_start)."""
+    root 2 xM / (B + sqrt(B^2 - 2 b xM^2)).  The stage kernels take it as a
+    face solve's result once the first Newton pass shows that the path ends
+    there (gpu-vah_amd/csrc/vah_stage.h, face_root)."""
     M0, M1, M2, M3 = q[0] - q[5], q[1] - q[6], q[2] - q[7], q[3] - q[8]
     PL = q[4]
     Ft = t * M3 / (M0 + PL)
@@ -116,6 +117,79 @@ def test_face_solve_root_in_closed_form(oracle_mod):
         worst_newton = max(worst_newton, abs(u - np.hypot(out[3], out[4])) / max(uT, 1e-3))
     assert worst_true < 1e-12, worst_true
     assert worst_newton < 1e-12, worst_newton
+
+
+def face_root_jump(t, q, u0):
+    """The stage kernels' test for a face solve started at u0 (face_root and
+    face_speeds in gpu-vah_amd/csrc/vah_stage.h): None when the path goes on
+    iterating, else the closed-form root the kernels take after the first
+    pass.  The first pass is the reference's update
+    (EnergyMomentumTensor.cpp:45-70); the path is then known to succeed when
+    the start lies left of the quadratic's vertex, the update is clearly
+    positive and e at the root is clearly positive."""
+    M0, M1, M2, M3 = q[0] - q[5], q[1] - q[6], q[2] - q[7], q[3] - q[8]
+    PL = q[4]
+    M0PL = M0 + PL if M0 + PL != 0 else 1e-16
+    Ft = t * M3 / M0PL
+    x = np.sqrt(abs(1 - Ft * Ft))
+    MBT = np.hypot(M1, M2) or 1e-16
+    xM, bL, b, c0 = x * MBT, x * x * M0 - Ft * Ft * PL, x * x + Ft * Ft, M0 - t * Ft * M3
+    B = bL + 0.5 * b * (c0 - PL)
+    disc = B * B - 2 * b * xM * xM
+    U = 2 * xM / (B + np.sqrt(max(disc, 0.0)))
+    ok = disc >= 0 and B > 0 and 0 <= U < 1 and c0 - U * xM > 1e-12 * abs(c0)
+    rs = 1 / np.sqrt(1 + u0 * u0)
+    U0 = u0 * rs
+    e = c0 - U0 * xM
+    g = bL + b * 0.5 * (e - PL)
+    fp = rs ** 3 * g - 0.5 * U0 * b * xM * rs ** 3
+    u1 = u0 - (U0 * g - xM) / (fp if fp != 0 else 1e-16)
+    if e < 0 or not (0 <= u1 <= 9.0072e15) or abs(u1 - u0) <= 1e-7 * abs(u1):
+        return None
+    if ok and U0 * b * xM <= B and u1 >= 1e-9 * u0:
+        return U / np.sqrt(1 - U * U)
+    return None
+
+
+def test_face_root_jump_keeps_the_reference_path(oracle_mod):
+    """Wherever the kernels' root jump fires, the oracle's Newton from the
+    same start succeeds and ends on that root: physical states with near and
+    far starts, near light-like flow, and random (often unphysical) conserved
+    variables.  A jump that skipped a failure the full path would meet would
+    show here as a failed orc_recover."""
+    L = oracle_mod.lib()
+    rng = np.random.default_rng(5)
+    out = np.zeros(6)
+    fired = 0
+    worst = 0.0
+    for k in range(20000):
+        t = rng.uniform(0.3, 5.0)
+        kind = k % 4
+        if kind == 2:
+            q = np.zeros(15)
+            q[0] = 10 ** rng.uniform(-3, 2)
+            q[1:4] = rng.normal(0, 1, 3) * q[0]
+            q[3] /= t
+            q[4] = q[0] * rng.uniform(-0.1, 1.0)
+            q[5:15] = rng.normal(0, 0.1, 10) * q[0]
+            u0 = 10 ** rng.uniform(-4, 1.5)
+        else:
+            e, pl, ut, ux, uy, un, pi = random_primitives(rng, t)
+            if kind == 3:
+                s = 10 ** rng.uniform(0.5, 1.5)
+                ux, uy = ux * s, uy * s
+                ut = np.sqrt(1 + ux * ux + uy * uy + t * t * un * un)
+            q = conserved_from_primitives(t, e, pl, ut, ux, uy, un, pi)
+            uT = np.hypot(ux, uy)
+            u0 = uT * rng.uniform(0.5, 1.5) if kind != 1 else 10 ** rng.uniform(-6, 3)
+        u = face_root_jump(t, q, u0)
+        if u is None:
+            continue
+        fired += 1
+        assert L.orc_recover(t, p(q), 1.0, u0, p(out)) == 0, (k, t, u0)
+        worst = max(worst, abs(u - np.hypot(out[3], out[4])) / max(u, 1e-300))
+    assert fired > 10000, fired
+    assert worst < 1e-10, worst
 
 
 def test_newton_fallback(oracle_mod):

@@ -192,16 +192,21 @@ __device__ __forceinline__ FaceRoot face_root(double t, const FaceState &S) {
 template <int DIR, bool ROOT_JUMP>
 __device__ __forceinline__ void face_speeds(double t, double guess, const FaceState &A, const FaceState &B, double &vA,
                                             double &vB) {
-    double ua = guess, ub = guess;
-    int sa = 0, sb = 0;
-    FaceRoot qa, qb;
-    double U0 = 0.0;
+    // the first pass, outside the loop: both sides start at `guess`
+    double ua, ub;
+    int sa = newton_update(t, A, guess, &ua);
+    int sb = newton_update(t, B, guess, &ub);
     if (ROOT_JUMP) {
-        qa = face_root(t, A);
-        qb = face_root(t, B);
-        U0 = guess * rsqrt(1.0 + guess * guess);
+        const FaceRoot qa = face_root(t, A), qb = face_root(t, B);
+        const double U0 = guess * rsqrt(1.0 + guess * guess);  // the first pass's U
+        const bool ja = (sa == 0) & qa.ok & (U0 * qa.bxM <= qa.B) & (ua >= 1.e-9 * guess);
+        const bool jb = (sb == 0) & qb.ok & (U0 * qb.bxM <= qb.B) & (ub >= 1.e-9 * guess);
+        ua = ja ? qa.u : ua;
+        sa = ja ? 1 : sa;
+        ub = jb ? qb.u : ub;
+        sb = jb ? 1 : sb;
     }
-    for (int it = 0; it < 10000 && (sa == 0 || sb == 0); ++it) {
+    for (int it = 1; it < 10000 && (sa == 0 || sb == 0); ++it) {
         double na, nb;
         const int ta = newton_update(t, A, ua, &na);
         const int tb = newton_update(t, B, ub, &nb);
@@ -210,14 +215,6 @@ __device__ __forceinline__ void face_speeds(double t, double guess, const FaceSt
         sa = ra ? ta : sa;
         ub = rb ? nb : ub;
         sb = rb ? tb : sb;
-        if (ROOT_JUMP) {
-            const bool ja = (it == 0) & (sa == 0) & qa.ok & (U0 * qa.bxM <= qa.B) & (ua >= 1.e-9 * guess);
-            const bool jb = (it == 0) & (sb == 0) & qb.ok & (U0 * qb.bxM <= qb.B) & (ub >= 1.e-9 * guess);
-            ua = ja ? qa.u : ua;
-            sa = ja ? 1 : sa;
-            ub = jb ? qb.u : ub;
-            sb = jb ? 1 : sb;
-        }
     }
     auto speed = [](const FaceState &S, double u, int st) {
         if (st != 1) return 0.0;

@@ -109,3 +109,34 @@ def test_seed_only_requests_are_served_like_the_single_lattice_path():
         print(f"request {r.rid}: {r.nx}x{r.ny}x{r.nz} from seed {r.glauber.seed}, frozen after {r.nstep} steps, "
               f"latency {r.latency * 1e3:.1f} ms, worst vs single lattice {err:.1e}")
         assert err <= 1e-10
+
+
+def test_serving_loop_retires_a_request_at_its_step_cap():
+    """A request still hot at its max_steps retires there (state
+    EVENT_STEP_LIMIT, exactly max_steps steps) with the single-lattice path's
+    fields for that many steps, while the others freeze out as usual: the cap
+    that bounds an event whose dilute edge turns unstable."""
+    V = vah()
+    S = importlib.import_module("gpu-vah_amd.serve")
+    tab = V.eos_lqcd_table(10000, 1e-8, 1e4)
+    e_peak, _ = V.eos_lqcd(0.175 / HBARC)
+    caps = [4, 3000, 6]
+    reqs = [S.Request(rid=i, arrival=0.0, nx=36, ny=40, nz=1, dx=0.1, dz=0.1, dt=0.02, t0=0.5, etabar=0.16,
+                      e=gaussian_ic(36, 40, seed=30 + i, peak=e_peak, floor=1e-4), max_steps=cap)
+            for i, cap in enumerate(caps)]
+    B = V.Batch(budget_bytes=1 << 30, max_events=4, eos_table=tab, freezeout_T_GeV=T_F, steps_per_graph=8)
+    out = S.serve(B, reqs, steps_per_poll=8, keep_fields=True)
+    B.close()
+    assert out["events"] == len(reqs) and out["rejected"] == 0
+    for r in reqs:
+        if r.max_steps < 100:
+            assert r.state == V.EVENT_STEP_LIMIT and r.nstep == r.max_steps, (r.rid, r.state, r.nstep)
+        else:
+            assert r.state == V.EVENT_FROZEN and 1 < r.nstep < r.max_steps
+        g = V.Vah(r.nx, r.ny, 1, r.dx, r.dx, r.dz, r.dt, r.etabar, r.t0, eos_table=tab)
+        g.set_primitives(r.e, *rest_frame(r.cells), pl=r.pl)
+        g.step(r.nstep)
+        err = max(field_errors(r.fields, g.state(), r.tau).values())
+        g.close()
+        print(f"request {r.rid}: cap {r.max_steps}, state {r.state} after {r.nstep} steps, worst {err:.1e}")
+        assert err <= 1e-12
